@@ -1,0 +1,211 @@
+/*
+ * sheen_c_api.h — C ABI of the MI355X render path (libsheen_c_api.so).
+ *
+ * The first block is the reference's binary contract, unchanged: the
+ * SheenScene layout (reference sheen_c_api.h:18-87) and the four exported
+ * functions (reference sheen_c_api.h:91-107, implemented in
+ * api/sheen_c_api.cu:160-365).  A caller that builds SheenScene for the
+ * reference library (Blender add-on engine.py:691-766, ctypes contract
+ * c_api_contract.py:233-274, CPython module api/sheen_py_module.cpp) can load
+ * this library instead without changes.
+ *
+ * The second block is additive: a resident-scene context for repeated
+ * renders, float radiance and primary-hit output for parity checks, ray
+ * counters and kernel timings, batched multi-scene rendering and the
+ * gradient (adjoint) entry points.  None of these exist in the reference.
+ *
+ * All pointers are host pointers unless a name says "device".  No HIP, CUDA
+ * or torch types appear in any signature; a stream is passed as void*.
+ */
+#ifndef SHEEN_C_API_H
+#define SHEEN_C_API_H
+
+#include "sheen_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reference sheen_c_api.h:18-87 — 352 bytes, photons@316, media@328. */
+typedef struct SheenScene {
+    CameraData camera;
+
+    const Object* objects;
+    int num_objects;
+
+    const SweepPoint* sweep_points;
+    int num_sweep_points;
+    const SweepStrand* sweep_strands;
+    int num_sweep_strands;
+
+    const Triangle* triangles;
+    int num_triangles;
+
+    const HeightField* heightfields;
+    int num_heightfields;
+
+    const Superellipsoid* superellipsoids;
+    int num_superellipsoids;
+
+    const LathePoint* lathe_points;
+    int num_lathe_points;
+    const Lathe* lathes;
+    int num_lathes;
+
+    const PrismPoint* prism_points;
+    int num_prism_points;
+    const Prism* prisms;
+    int num_prisms;
+
+    const CSGNode* csg_nodes;
+    int csg_root;
+    int num_csg_nodes;
+
+    const Plane* planes;
+    int num_planes;
+
+    const Material* materials;
+    int num_materials;
+
+    const unsigned char* const* texture_pixels; /* [num_textures], w*h*3 RGB, top row first */
+    const int* texture_widths;
+    const int* texture_heights;
+    int num_textures;
+
+    const Light* lights;
+    int num_lights;
+
+    int photons;        /* photons emitted per light; 0 disables the caustic path */
+    float photon_cell;  /* hash-grid cell size; <= 0 selects 0.25 */
+    float caustics;     /* caustic gather scale; 0 disables the gather */
+
+    const Medium* media;
+    int num_media;
+} SheenScene;
+
+/* ------------------------------------------------------------------------- */
+/* Reference entry points                                                     */
+/* ------------------------------------------------------------------------- */
+
+/* Replaces reference sheen_c_api.h:91 / api/sheen_c_api.cu:160-180.
+ * Selects the HIP device; cuda_arch is accepted and ignored.  0 or -1. */
+int sheen_init(int device_id, int cuda_arch);
+
+/* Replaces reference sheen_c_api.h:94 / api/sheen_c_api.cu:182-185.
+ * Synchronises and releases every device buffer the library holds; call
+ * sheen_init again before the next render. */
+void sheen_cleanup(void);
+
+/* Replaces reference sheen_c_api.h:99-104 / api/sheen_c_api.cu:187-361.
+ * Synchronous: builds the BVH, uploads, renders and writes an 8-bit sRGB RGB
+ * PNG to output_path.  Validation as the reference: scene != NULL,
+ * width,height > 0, 1 <= aa_samples <= 64, output_path != NULL,
+ * num_materials > 0.  Returns 0, or -1 with sheen_get_last_error() set. */
+int sheen_render_scene(const SheenScene* scene, int width, int height,
+                       int aa_samples, const char* output_path);
+
+/* Replaces reference sheen_c_api.h:107 / api/sheen_c_api.cu:363-365.
+ * Static buffer, valid until the next call into the library. */
+const char* sheen_get_last_error(void);
+
+/* ------------------------------------------------------------------------- */
+/* Extensions (no counterpart in the reference)                               */
+/* ------------------------------------------------------------------------- */
+
+/* Ray counters and timings of the last render on a context.  A "ray" is one
+ * traversal call: primary, shadow pass (each glass pass counts), reflection,
+ * refraction face, photon bounce. */
+typedef struct SheenRayStats {
+    uint64_t primary_rays;
+    uint64_t shadow_rays;
+    uint64_t reflection_rays;
+    uint64_t refraction_rays;
+    uint64_t photon_rays;
+    uint64_t total_rays;
+    uint64_t samples;
+    /* Filled only with counting enabled (sheen_context_set_counting): wide
+     * BVH nodes fetched and leaf primitives tested, over all rays. */
+    uint64_t node_fetches;
+    uint64_t prim_tests;
+    uint64_t shadow_node_fetches;
+    uint64_t shadow_prim_tests;
+    /* GPU time from the first to the last kernel of the frame (hipEvents). */
+    float frame_ms;
+    /* With profiling enabled: summed durations of the closest-hit trace
+     * kernels and of the shadow trace kernel (hipEvents on the render stream). */
+    float trace_ms;
+    float shadow_ms;
+    float shade_ms;
+    int trace_launches;
+    int shadow_launches;
+} SheenRayStats;
+
+typedef struct SheenContext SheenContext;
+
+/* Builds the wide BVHs on the host and uploads the scene to the current
+ * device.  The scene arrays are copied; the caller keeps ownership.
+ * Returns NULL with sheen_get_last_error() set on failure. */
+SheenContext* sheen_context_create(const SheenScene* scene);
+void sheen_context_destroy(SheenContext* ctx);
+
+/* Renders one frame.  out_rgba: host float[width*height*4] or NULL (the
+ * frame then stays in device memory, see sheen_context_frame_device).
+ * stats may be NULL.  Synchronous; 0 or -1. */
+int sheen_context_render(SheenContext* ctx, int width, int height, int aa_samples,
+                         float* out_rgba, SheenRayStats* stats);
+
+/* Device pointer to the last frame: float4 per pixel, row-major. */
+const float* sheen_context_frame_device(const SheenContext* ctx);
+
+/* Per-sample primary hits of a frame: kind[i], index[i], t[i] for
+ * i = (y*width + x)*aa + s.  kind: -1 miss, 0 sphere, 1 strand, 2 triangle,
+ * 3 heightfield, 4 superellipsoid, 5 lathe, 6 prism, 7 CSG, 8 plane
+ * (the reference's category order, integrator/raycast.h:33-68).  Host arrays. */
+int sheen_context_primary_hits(SheenContext* ctx, int width, int height, int aa_samples,
+                               int* kind, int* index, float* t);
+
+/* 1 = record hipEvents around every kernel (SheenRayStats.*_ms). */
+int sheen_context_set_profiling(SheenContext* ctx, int enable);
+/* 1 = count node fetches / primitive tests (slower; for roofline bytes). */
+int sheen_context_set_counting(SheenContext* ctx, int enable);
+/* Samples per wavefront chunk (0 = automatic). */
+int sheen_context_set_chunk(SheenContext* ctx, int samples_per_chunk);
+
+/* Photon pass results of a context (scene.photons > 0): photons emitted and
+ * stored (reference render_kernel.cu:928-953). */
+int sheen_context_photon_stats(SheenContext* ctx, int* emitted, int* stored);
+/* Copies the cell-sorted photon map: photons[stored], cell_start[2^18],
+ * cell_count[2^18] (any may be NULL).  Returns the photon count or -1. */
+int sheen_context_photon_map(SheenContext* ctx, Photon* photons, int* cell_start, int* cell_count);
+/* Gather counts around every stride-th stored photon, through the hash map
+ * (brute = 0) or by brute force over all photons (brute = 1); reference
+ * render_kernel.cu:981-990 and 1092-1107.  Returns the probe count or -1. */
+int sheen_context_photon_probe(SheenContext* ctx, int stride, float radius, int max_samples, int brute,
+                               int* counts);
+
+/* One-shot float render: create, render, download, destroy. */
+int sheen_render_float(const SheenScene* scene, int width, int height, int aa_samples,
+                       float* out_rgba, SheenRayStats* stats);
+
+/* Linear -> sRGB 8-bit encoding of a float RGBA frame, as written to PNG
+ * (reference png_color.h:11-21).  out_rgb: width*height*3 bytes. */
+void sheen_encode_srgb(const float* rgba, int width, int height, unsigned char* out_rgb);
+
+/* Writes an 8-bit RGB PNG.  0 or -1. */
+int sheen_write_png(const char* path, int width, int height, const unsigned char* rgb);
+
+#ifdef __cplusplus
+} /* extern "C" */
+SHEEN_STATIC_ASSERT(sizeof(SheenScene) == 352, "SheenScene 352");
+SHEEN_STATIC_ASSERT(offsetof(SheenScene, photons) == 316, "SheenScene.photons@316");
+SHEEN_STATIC_ASSERT(offsetof(SheenScene, photon_cell) == 320, "SheenScene.photon_cell@320");
+SHEEN_STATIC_ASSERT(offsetof(SheenScene, caustics) == 324, "SheenScene.caustics@324");
+SHEEN_STATIC_ASSERT(offsetof(SheenScene, media) == 328, "SheenScene.media@328");
+SHEEN_STATIC_ASSERT(offsetof(SheenScene, num_media) == 336, "SheenScene.num_media@336");
+#else
+SHEEN_STATIC_ASSERT(sizeof(SheenScene) == 352, "SheenScene 352");
+SHEEN_STATIC_ASSERT(offsetof(SheenScene, photons) == 316, "SheenScene.photons@316");
+SHEEN_STATIC_ASSERT(offsetof(SheenScene, media) == 328, "SheenScene.media@328");
+#endif
+
+#endif /* SHEEN_C_API_H */

@@ -1,0 +1,71 @@
+"""Build recipe of the native libraries (in-tree, gfx950).
+
+libsheen_host.so   host C++: scene generators, wide-BVH builder, PNG/sRGB.
+libsheen_c_api.so  HIP kernels + the C ABI of include/sheen_c_api.h.
+_sheen*.so         CPython module `sheen_amd._sheen` (reference
+                   api/sheen_py_module.cpp counterpart) over libsheen_c_api.so.
+"""
+import os
+import subprocess
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INC = os.path.join(ROOT, "include")
+OBJ = os.path.join(HERE, "_obj")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("SHEEN_OFFLOAD_ARCH", "gfx950")
+
+HOST_SRCS = ["wbvh_build.cpp", "host_scene.cpp", "png.cpp"]
+CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fwrapv", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
+HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "--offload-arch=" + ARCH,
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def _newer(out, deps):
+    if not os.path.exists(out):
+        return True
+    t = os.path.getmtime(out)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _headers():
+    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    hs += [os.path.join(INC, f) for f in os.listdir(INC) if f.endswith(".h")]
+    return hs
+
+
+def build(force=True):
+    os.makedirs(OBJ, exist_ok=True)
+    hdrs = _headers()
+    objs = []
+    for src in HOST_SRCS + ["scenes.cpp", "host_api.cpp"]:
+        o = os.path.join(OBJ, src.replace(".cpp", ".o"))
+        if force or _newer(o, [os.path.join(CSRC, src)] + hdrs):
+            _run(["g++"] + CXXFLAGS + ["-I", INC, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
+    host_lib = os.path.join(HERE, "libsheen_host.so")
+    _run(["g++", "-shared", "-o", host_lib] + objs + ["-lz"])
+    render_o = os.path.join(OBJ, "render.o")
+    if force or _newer(render_o, [os.path.join(CSRC, "render.hip")] + hdrs):
+        _run([HIPCC] + HIPFLAGS + ["-I", INC, "-I", CSRC, "-c", os.path.join(CSRC, "render.hip"), "-o", render_o])
+    capi = os.path.join(HERE, "libsheen_c_api.so")
+    host_objs = [os.path.join(OBJ, s.replace(".cpp", ".o")) for s in HOST_SRCS]
+    _run([HIPCC, "-shared", "--offload-arch=" + ARCH, "-o", capi, render_o] + host_objs + ["-lz"])
+    # CPython module
+    pyinc = sysconfig.get_paths()["include"]
+    ext = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    mod = os.path.join(HERE, "_sheen" + ext)
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", INC, "-I", pyinc, os.path.join(CSRC, "py_sheen.cpp"),
+          "-o", mod, "-L", HERE, "-lsheen_c_api", "-Wl,-rpath,$ORIGIN"])
+    return [host_lib, capi, mod]
+
+
+if __name__ == "__main__":
+    build()

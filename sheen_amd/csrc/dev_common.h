@@ -1,0 +1,109 @@
+// dev_common.h — vector math, RNG and the device scene view of the HIP path.
+//
+// All device arithmetic is compiled with contraction off: every product and
+// sum rounds separately, in the reference's operation order, so that hit
+// distances are reproducible bit for bit on the CPU.  normalize(v) is
+// v * (1/sqrt(v.v)) with correctly rounded sqrt and division.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sheen_c_api.h"
+#include "wbvh.h"
+
+#define SHD __device__ __forceinline__
+#define SHHD __host__ __device__ __forceinline__
+
+struct v3 { float x, y, z; };
+SHHD v3 mk3(float x, float y, float z) { return {x, y, z}; }
+SHHD v3 cv3(const sheen_f3& f) { return {f.x, f.y, f.z}; }
+SHHD v3 operator+(v3 a, v3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+SHHD v3 operator-(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+SHHD v3 operator-(v3 a) { return {-a.x, -a.y, -a.z}; }
+SHHD v3 operator*(v3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+SHHD v3 operator*(float s, v3 a) { return {a.x * s, a.y * s, a.z * s}; }
+SHHD v3 operator*(v3 a, v3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+SHHD v3 operator/(v3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+SHHD float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+SHHD v3 cross(v3 a, v3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+SHHD v3 normalize(v3 v) {
+    float inv = 1.0f / sqrtf(dot(v, v));
+    return {v.x * inv, v.y * inv, v.z * inv};
+}
+
+// Integer hash and uniform draws (reference sampling/aa.h:32-41).
+SHHD uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+SHHD float rand01(uint32_t seed) { return float(mix32(seed) & 0xffffffu) / 16777216.0f; }
+
+// Primitive categories in the reference's closest-hit order
+// (integrator/raycast.h:33-68).
+enum : int { C_SPHERE = 0, C_STRAND = 1, C_TRI = 2, C_HF = 3, C_SE = 4, C_LATHE = 5, C_PRISM = 6, C_CSG = 7, C_PLANE = 8 };
+#define N_BVH_CATS 7
+
+struct PhotonMapView {
+    const Photon* photons;  // sorted by cell
+    const int* cell_start;
+    const int* cell_count;
+    int num_cells;
+    float cell;
+};
+
+// Device scene: everything a kernel reads, passed by value.
+struct DScene {
+    const WNode* bvh[N_BVH_CATS];
+    const uint32_t* prim_ids[N_BVH_CATS];  // leaf slot -> primitive index
+    const float* bounds[N_BVH_CATS];       // 6 floats per primitive (reference bounds rules); tri: unused
+    int count[N_BVH_CATS];
+    int cat_opaque[N_BVH_CATS];            // 1 = no material of the category can pass a shadow ray
+    const float4* tri_leaf;                // 3 float4 per triangle leaf slot: v0|idx, v1, v2
+
+    const Object* objects;
+    const SweepStrand* strands;
+    const SweepPoint* sweep_points;
+    const Triangle* triangles;
+    int num_triangles;
+    const HeightField* hfs;
+    const Superellipsoid* ses;
+    const Lathe* lathes;
+    const LathePoint* lathe_points;
+    const Prism* prisms;
+    const PrismPoint* prism_points;
+    const CSGNode* csg;
+    int csg_root;
+    int num_csg;
+    const Plane* planes;
+    int num_planes;
+    const Material* materials;
+    int num_materials;
+    const TextureRef* textures;
+    int num_textures;
+    const Light* lights;
+    int num_lights;
+    const Medium* media;
+    int num_media;
+    PhotonMapView pm;
+    float caustics;
+    int any_strands;
+    int any_beams;
+    CameraData cam;
+};
+
+struct DRay {
+    v3 o, d;
+    float tmin, tmax;
+};
+
+// Hit record (reference types.h:34-44 semantics; kind/index added).
+struct DHit {
+    float t;
+    int material_id;
+    int object_id;
+    int kind, index;
+    v3 point, normal, tangent;
+    float uvx, uvy;
+};

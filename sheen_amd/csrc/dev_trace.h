@@ -1,0 +1,467 @@
+// dev_trace.h — wide-BVH traversal and scene queries of the HIP path.
+//
+// Semantics follow the reference's scene queries (integrator/raycast.h:14-97
+// over the per-category traversals bvh.h:605-922):
+//   * a primitive is a candidate when its own bounds pass the reference's
+//     slab test (the gate every leaf passes in the reference traversal);
+//   * closest hit within a category; equal distances go to the lowest
+//     primitive index (the reference's winner depends on its DFS order);
+//   * across categories the earlier category wins ties (strict <);
+//   * the opaque query returns the closest hit of the first category that
+//     has any hit (planes last).
+// The traversal visits a superset of the nodes the reference visits: node
+// boxes decode conservatively, the slab arithmetic is the reference's
+// (b - o) * (1/d), and closest-distance culling keeps a relative margin of
+// 2^-7 so a primitive whose computed distance undershoots its box entry by
+// rounding is still tested.
+#pragma once
+#include "dev_prims.h"
+
+#define LDS_DEPTH 16
+#define SPILL_DEPTH 112
+#define TRACE_BLOCK 256
+#define CULL_MARGIN 0.0078125f
+
+// 19-comparator sorting network for 8 keys (ascending).
+SHD void cswap(uint32_t& a, uint32_t& b) {
+    uint32_t lo = min(a, b), hi = max(a, b);
+    a = lo; b = hi;
+}
+SHD void sort8(uint32_t* k) {
+    cswap(k[0], k[2]); cswap(k[1], k[3]); cswap(k[4], k[6]); cswap(k[5], k[7]);
+    cswap(k[0], k[4]); cswap(k[1], k[5]); cswap(k[2], k[6]); cswap(k[3], k[7]);
+    cswap(k[0], k[1]); cswap(k[2], k[3]); cswap(k[4], k[5]); cswap(k[6], k[7]);
+    cswap(k[2], k[4]); cswap(k[3], k[5]);
+    cswap(k[1], k[4]); cswap(k[3], k[6]);
+    cswap(k[1], k[2]); cswap(k[3], k[4]); cswap(k[5], k[6]);
+}
+
+// Per-lane traversal stack: the top LDS_DEPTH entries live in LDS (one
+// column per lane, conflict-free), older entries spill to private memory.
+struct TStack {
+    uint2* lds;  // &shared[tid], stride TRACE_BLOCK
+    uint2 spill[SPILL_DEPTH];
+    int sp;
+    SHD void push(uint32_t node, float t) {
+        int slot = sp % LDS_DEPTH;
+        if (sp >= LDS_DEPTH) spill[sp - LDS_DEPTH] = lds[slot * TRACE_BLOCK];
+        lds[slot * TRACE_BLOCK] = make_uint2(node, __float_as_uint(t));
+        sp++;
+    }
+    SHD uint2 pop() {
+        sp--;
+        int slot = sp % LDS_DEPTH;
+        uint2 e = lds[slot * TRACE_BLOCK];
+        if (sp >= LDS_DEPTH) lds[slot * TRACE_BLOCK] = spill[sp - LDS_DEPTH];
+        return e;
+    }
+};
+
+struct TraceStats {
+    uint32_t nodes, prims;
+};
+
+// Leaf test of one primitive reference: reference gate, then the distance.
+template <int CAT>
+SHD bool leaf_dist(const DScene& S, uint32_t slot, const DRay& r, v3 inv, float& t, int& idx) {
+    if (CAT == C_TRI) {
+        float4 a = S.tri_leaf[3 * slot], b = S.tri_leaf[3 * slot + 1], c = S.tri_leaf[3 * slot + 2];
+        idx = __float_as_int(a.w);
+        float mn[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z))};
+        float mx[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z))};
+        if (!ref_slab(r, inv, mn, mx)) return false;
+        return t_triangle(r, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t);
+    }
+    idx = (int)S.prim_ids[CAT][slot];
+    const float* bb = S.bounds[CAT] + 6 * (size_t)idx;
+    if (!ref_slab(r, inv, bb, bb + 3)) return false;
+    DHit h;
+    switch (CAT) {
+        case C_SPHERE: {
+            const Object& o = S.objects[idx];
+            return t_sphere(r, cv3(o.param1), o.param2, t);
+        }
+        case C_STRAND:
+            if (!hit_strand(r, S.strands[idx], S.sweep_points, h)) return false;
+            break;
+        case C_HF:
+            if (!hit_hf(r, S.hfs[idx], h)) return false;
+            break;
+        case C_SE:
+            if (!hit_se(r, S.ses[idx], h)) return false;
+            break;
+        case C_LATHE:
+            if (!hit_lathe(r, S.lathes[idx], S.lathe_points, h)) return false;
+            break;
+        case C_PRISM:
+            if (!hit_prism(r, S.prisms[idx], S.prism_points, h)) return false;
+            break;
+        default:
+            return false;
+    }
+    t = h.t;
+    return true;
+}
+
+// Closest (or, with ANY, first) hit of one category's wide BVH.
+// cat_t: in = acceptance bound (exclusive unless a same-category tie), out =
+// best distance; cat_idx/cat_hit as the running best of this category.
+template <int CAT, bool ANY, bool COUNT>
+SHD void traverse(const DScene& S, const DRay& r, v3 inv, float& cat_t, int& cat_idx, bool& cat_hit, TStack& st,
+                  TraceStats& ts) {
+    const WNode* nodes = S.bvh[CAT];
+    st.sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
+        uint4 h0 = np[0], h1 = np[1];
+        uint4 q0 = np[2], q1 = np[3], q2 = np[4], q3 = np[5], q4 = np[6], q5 = np[7];
+        if (COUNT) ts.nodes++;
+        const float ox = __uint_as_float(h0.x), oy = __uint_as_float(h0.y), oz = __uint_as_float(h0.z);
+        const float sx = __uint_as_float((h0.w & 0xFFu) << 23);
+        const float sy = __uint_as_float(((h0.w >> 8) & 0xFFu) << 23);
+        const float sz = __uint_as_float(((h0.w >> 16) & 0xFFu) << 23);
+        const uint32_t child_base = h1.x, prim_base = h1.y;
+        const uint64_t meta = (uint64_t)h1.z | ((uint64_t)h1.w << 32);
+        const float bound = cat_t + cat_t * CULL_MARGIN;
+        uint32_t keys[8];
+        uint32_t leafmask = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
+            const int w = i >> 1, sh = 16 * (i & 1);
+            auto g = [&](uint4 row) -> float {
+                uint32_t d = (w == 0) ? row.x : (w == 1) ? row.y : (w == 2) ? row.z : row.w;
+                return (float)((d >> sh) & 0xFFFFu);
+            };
+            const float lx = fmaf(g(q0), sx, ox), hx = fmaf(g(q1), sx, ox);
+            const float ly = fmaf(g(q2), sy, oy), hy = fmaf(g(q3), sy, oy);
+            const float lz = fmaf(g(q4), sz, oz), hz = fmaf(g(q5), sz, oz);
+            const float ax = (lx - r.o.x) * inv.x, bx = (hx - r.o.x) * inv.x;
+            const float ay = (ly - r.o.y) * inv.y, by = (hy - r.o.y) * inv.y;
+            const float az = (lz - r.o.z) * inv.z, bz = (hz - r.o.z) * inv.z;
+            const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), r.tmin));
+            const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), bound));
+            const bool hit = (m != WBVH_EMPTY) && (tn <= tf);
+            const bool leaf = (m >> 5) != 0;
+            keys[i] = (hit && !leaf) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
+            if (hit && leaf) leafmask |= 1u << i;
+        }
+        while (leafmask) {
+            const int i = __builtin_ctz(leafmask);
+            leafmask &= leafmask - 1;
+            const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
+            const uint32_t cnt = m >> 5, off = m & 31u;
+            for (uint32_t k = 0; k < cnt; k++) {
+                float t;
+                int idx;
+                if (COUNT) ts.prims++;
+                if (leaf_dist<CAT>(S, prim_base + off + k, r, inv, t, idx)) {
+                    if (t < cat_t || (t == cat_t && cat_hit && idx < cat_idx)) {
+                        cat_t = t;
+                        cat_idx = idx;
+                        cat_hit = true;
+                        if (ANY) return;
+                    }
+                }
+            }
+        }
+        sort8(keys);
+        if (keys[0] != 0xFFFFFFFFu) {
+#pragma unroll
+            for (int j = 7; j >= 1; j--) {
+                if (keys[j] != 0xFFFFFFFFu) {
+                    const uint32_t m = (uint32_t)(meta >> (8 * (keys[j] & 7u))) & 0xFFu;
+                    if (st.sp < LDS_DEPTH + SPILL_DEPTH) st.push(child_base + (m & 31u), __uint_as_float(keys[j] & ~7u));
+                }
+            }
+            const uint32_t m0 = (uint32_t)(meta >> (8 * (keys[0] & 7u))) & 0xFFu;
+            node = child_base + (m0 & 31u);
+            continue;
+        }
+        bool found = false;
+        while (st.sp > 0) {
+            uint2 e = st.pop();
+            if (__uint_as_float(e.y) <= cat_t + cat_t * CULL_MARGIN) {
+                node = e.x;
+                found = true;
+                break;
+            }
+        }
+        if (!found) return;
+    }
+}
+
+// CSG tree evaluation (reference integrator/csg.h:113-186) with an explicit
+// post-order stack.  Returns the hit distance and the interval record.
+SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& best_iv, float& t_out) {
+    struct Frame { int node; int state; int nA; CSpan A[8]; };
+    Frame fr[CSG_MAX_DEPTH];
+    CSpan ret[8];
+    int nret = 0;
+    int top = 0;
+    fr[0].node = S.csg_root; fr[0].state = 0; fr[0].nA = 0;
+    while (top >= 0) {
+        Frame& f = fr[top];
+        const CSGNode& n = S.csg[f.node];
+        if (n.leaf_kind >= 0) {
+            nret = 0;
+            CSpan iv;
+            iv.te = -1e30f; iv.tx = -1e30f;
+            bool ok = false;
+            if (n.leaf_kind == 0) {
+                const Object& o = S.objects[n.leaf_index];
+                ok = sphere_span(r, cv3(o.param1), o.param2, t0, t1, iv.te, iv.tx);
+            } else if (n.leaf_kind == 1) {
+                ok = se_span(r, S.ses[n.leaf_index], t0, t1, iv.te, iv.tx);
+            }
+            if (ok) {
+                iv.mat = (n.leaf_kind == 0) ? S.objects[n.leaf_index].material_id
+                       : (n.leaf_kind == 1) ? S.ses[n.leaf_index].material_id : 0;
+                iv.leaf = n.leaf_index;
+                iv.kind = n.leaf_kind;
+                ret[nret++] = iv;
+            }
+            top--;
+            continue;
+        }
+        if (f.state == 0) {  // evaluate left
+            f.state = 1;
+            if (n.left >= 0 && top + 1 < CSG_MAX_DEPTH) {
+                fr[top + 1].node = n.left; fr[top + 1].state = 0; fr[top + 1].nA = 0;
+                top++;
+            } else {
+                nret = 0;
+            }
+            continue;
+        }
+        if (f.state == 1) {  // left result in ret; evaluate right
+            f.nA = nret;
+            for (int i = 0; i < nret; i++) f.A[i] = ret[i];
+            f.state = 2;
+            if (n.right >= 0 && top + 1 < CSG_MAX_DEPTH) {
+                fr[top + 1].node = n.right; fr[top + 1].state = 0; fr[top + 1].nA = 0;
+                top++;
+            } else {
+                nret = 0;
+            }
+            continue;
+        }
+        // combine left (f.A) with right (ret)
+        CSpan B[8];
+        int nB = nret;
+        for (int i = 0; i < nB; i++) B[i] = ret[i];
+        csg_combine(f.A, f.nA, B, nB, n.op, ret, nret);
+        top--;
+    }
+    float best = 1e30f;
+    int bi = -1;
+    for (int i = 0; i < nret; i++)
+        if (ret[i].te < best && ret[i].tx > t0) { best = ret[i].te; bi = i; }
+    if (bi < 0) return false;
+    float t = fmaxf(ret[bi].te, t0);
+    if (t < t0 || t > t1) return false;
+    best_iv = ret[bi];
+    t_out = t;
+    return true;
+}
+
+struct QHit {
+    float t;
+    int kind;   // -1 miss
+    int idx;
+    int carry;  // object_id carried into a CSG hit
+};
+
+template <int CAT, bool COUNT>
+SHD void closest_cat(const DScene& S, const DRay& r, v3 inv, float init_t, QHit& q, float& best, int& last_obj,
+                     TStack& st, TraceStats& ts) {
+    if (S.count[CAT] == 0) return;
+    float ct = init_t;
+    int ci = -1;
+    bool ch = false;
+    traverse<CAT, false, COUNT>(S, r, inv, ct, ci, ch, st, ts);
+    if (ch) {
+        last_obj = (CAT == C_SPHERE) ? ci : -(ci + 1);
+        if (ct < best) { best = ct; q.t = ct; q.kind = CAT; q.idx = ci; }
+    }
+}
+
+// scene_intersect (reference integrator/raycast.h:14-70).
+template <bool COUNT>
+SHD void query_closest(const DScene& S, const DRay& r, QHit& q, TStack& st, TraceStats& ts) {
+    const v3 inv = mk3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    q.t = 0.0f; q.kind = -1; q.idx = -1; q.carry = 0;
+    float best = r.tmax;
+    int last_obj = 0;
+    // With a CSG tree over a scene with triangles the CSG hit inherits the
+    // object id of the last category that hit at all, so those categories
+    // must not be culled by the running best.
+    const bool exact_carry = (S.csg_root >= 0) && (S.num_triangles > 0);
+#define SHEEN_INIT (exact_carry ? r.tmax : best)
+    closest_cat<C_SPHERE, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+    closest_cat<C_STRAND, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+    closest_cat<C_TRI, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+    closest_cat<C_HF, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+    closest_cat<C_SE, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+    closest_cat<C_LATHE, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+    closest_cat<C_PRISM, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+#undef SHEEN_INIT
+    if (S.csg_root >= 0) {
+        CSpan iv;
+        float t;
+        if (csg_query(S, r, r.tmin, r.tmax, iv, t) && t < best) {
+            best = t; q.t = t; q.kind = C_CSG; q.idx = 0; q.carry = last_obj;
+        }
+    }
+    for (int i = 0; i < S.num_planes; i++) {
+        DHit h;
+        const Plane& p = S.planes[i];
+        if (hit_plane(r, cv3(p.normal), p.d, h) && h.t < best) { best = h.t; q.t = h.t; q.kind = C_PLANE; q.idx = i; }
+    }
+}
+
+template <int CAT, bool COUNT>
+SHD bool opaque_cat(const DScene& S, const DRay& r, v3 inv, QHit& q, TStack& st, TraceStats& ts) {
+    if (S.count[CAT] == 0) return false;
+    float ct = r.tmax;
+    int ci = -1;
+    bool ch = false;
+    if (S.cat_opaque[CAT]) traverse<CAT, true, COUNT>(S, r, inv, ct, ci, ch, st, ts);
+    else traverse<CAT, false, COUNT>(S, r, inv, ct, ci, ch, st, ts);
+    if (ch) { q.t = ct; q.kind = CAT; q.idx = ci; }
+    return ch;
+}
+
+// scene_intersect_opaque (reference integrator/raycast.h:74-97).  For a
+// category whose materials can never pass a shadow ray the first hit found
+// decides (the closest hit of that category would block as well).
+template <bool COUNT>
+SHD bool query_opaque(const DScene& S, const DRay& r, QHit& q, TStack& st, TraceStats& ts) {
+    const v3 inv = mk3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    q.kind = -1; q.idx = -1; q.carry = 0; q.t = 0.0f;
+    if (opaque_cat<C_SPHERE, COUNT>(S, r, inv, q, st, ts)) return true;
+    if (opaque_cat<C_TRI, COUNT>(S, r, inv, q, st, ts)) return true;
+    if (opaque_cat<C_HF, COUNT>(S, r, inv, q, st, ts)) return true;
+    if (opaque_cat<C_SE, COUNT>(S, r, inv, q, st, ts)) return true;
+    if (opaque_cat<C_LATHE, COUNT>(S, r, inv, q, st, ts)) return true;
+    if (opaque_cat<C_PRISM, COUNT>(S, r, inv, q, st, ts)) return true;
+    if (S.csg_root >= 0) {
+        CSpan iv;
+        float t;
+        if (csg_query(S, r, r.tmin, r.tmax, iv, t)) { q.t = t; q.kind = C_CSG; q.idx = 0; q.carry = 0; return true; }
+    }
+    bool any = false;
+    float closest = r.tmax;
+    for (int i = 0; i < S.num_planes; i++) {
+        DHit h;
+        const Plane& p = S.planes[i];
+        if (hit_plane(r, cv3(p.normal), p.d, h) && h.t < closest) { closest = h.t; q.t = h.t; q.kind = C_PLANE; q.idx = i; any = true; }
+    }
+    return any;
+}
+
+// Material of a hit without rebuilding the record.
+SHD int hit_material(const DScene& S, const QHit& q, const DRay& r) {
+    switch (q.kind) {
+        case C_SPHERE: return S.objects[q.idx].material_id;
+        case C_STRAND: return S.strands[q.idx].material_id;
+        case C_TRI: return S.triangles[q.idx].material_id;
+        case C_HF: return S.hfs[q.idx].material_id;
+        case C_SE: return S.ses[q.idx].material_id;
+        case C_LATHE: return S.lathes[q.idx].material_id;
+        case C_PRISM: return S.prisms[q.idx].material_id;
+        case C_CSG: {
+            CSpan iv;
+            float t;
+            csg_query(S, r, r.tmin, r.tmax, iv, t);
+            return iv.mat;
+        }
+        case C_PLANE: return S.planes[q.idx].material_id;
+    }
+    return 0;
+}
+
+// Full hit record of the winning primitive: its test is run again on the
+// same ray, which reproduces the distance and every attribute exactly.
+SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
+    h.kind = q.kind;
+    h.index = q.idx;
+    h.tangent = mk3(0, 0, 0);
+    h.uvx = 0.0f; h.uvy = 0.0f;
+    switch (q.kind) {
+        case C_SPHERE: {
+            const Object& o = S.objects[q.idx];
+            hit_sphere(r, cv3(o.param1), o.param2, h);
+            h.material_id = o.material_id; h.object_id = q.idx;
+            break;
+        }
+        case C_STRAND:
+            hit_strand(r, S.strands[q.idx], S.sweep_points, h);
+            h.object_id = -(q.idx + 1);
+            break;
+        case C_TRI:
+            hit_triangle(r, S.triangles[q.idx], h);
+            h.material_id = S.triangles[q.idx].material_id; h.object_id = -(q.idx + 1);
+            break;
+        case C_HF:
+            hit_hf(r, S.hfs[q.idx], h);
+            h.material_id = S.hfs[q.idx].material_id; h.object_id = -(q.idx + 1);
+            break;
+        case C_SE:
+            hit_se(r, S.ses[q.idx], h);
+            h.material_id = S.ses[q.idx].material_id; h.object_id = -(q.idx + 1);
+            break;
+        case C_LATHE:
+            hit_lathe(r, S.lathes[q.idx], S.lathe_points, h);
+            h.object_id = -(q.idx + 1);
+            break;
+        case C_PRISM:
+            hit_prism(r, S.prisms[q.idx], S.prism_points, h);
+            h.object_id = -(q.idx + 1);
+            break;
+        case C_CSG: {
+            CSpan iv;
+            float t;
+            csg_query(S, r, r.tmin, r.tmax, iv, t);
+            h.t = t;
+            h.material_id = iv.mat;
+            h.point = r.o + t * r.d;
+            if (iv.kind == 0) {
+                h.normal = normalize(h.point - cv3(S.objects[iv.leaf].param1));
+                h.tangent = mk3(0, 0, 0);
+            } else if (iv.kind == 1) {
+                h.normal = normalize(se_gradient(S.ses[iv.leaf], h.point));
+                h.tangent = mk3(1, 0, 0);
+            } else {
+                h.normal = mk3(0, 1, 0);
+                h.tangent = mk3(1, 0, 0);
+            }
+            h.uvx = h.point.x; h.uvy = h.point.y;
+            if (dot(h.normal, r.d) > 0.0f) h.normal = -h.normal;
+            h.object_id = q.carry;
+            break;
+        }
+        case C_PLANE: {
+            const Plane& p = S.planes[q.idx];
+            hit_plane(r, cv3(p.normal), p.d, h);
+            h.material_id = p.material_id; h.object_id = -1 - q.idx;
+            break;
+        }
+    }
+}
+
+// Optical depth of all strands along a shadow ray (reference bvh.h:928-960):
+// brute force over strands through the strand BVH's leaf order.
+SHD float strands_depth(const DScene& S, const DRay& r) {
+    if (S.count[C_STRAND] == 0) return 0.0f;
+    const v3 inv = mk3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    float tau = 0.0f;
+    for (int i = 0; i < S.count[C_STRAND]; i++) {
+        const float* bb = S.bounds[C_STRAND] + 6 * (size_t)i;
+        if (!ref_slab(r, inv, bb, bb + 3)) continue;
+        tau += strand_depth(r, S.strands[i], S.sweep_points, 6.0f);
+        if (tau > 6.0f) return tau;
+    }
+    return tau;
+}

@@ -1,0 +1,57 @@
+// host_util.h — host-side scene preparation shared by the C ABI library and
+// the host tools library: reference bounds rules, wide-BVH builds per
+// category, triangle leaf copies, texture mip chains, validation.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "sheen_c_api.h"
+#include "wbvh.h"
+
+struct HostBvh {
+    std::vector<WNode> nodes;
+    std::vector<uint32_t> prims;
+    std::vector<float> bounds;  // 6 per primitive (all primitives of the category)
+    int count = 0;
+    int opaque = 1;
+    int max_stack = 0;
+    int depth = 0;
+};
+
+struct HostTexture {
+    std::vector<unsigned char> px;  // base level + mip chain, RGB
+    int w = 0, h = 0, mips = 1;
+};
+
+struct HostSceneInfo {
+    uint64_t photon_rays = 0;
+    int photons_emitted = 0;
+    int photons_stored = 0;
+    double bvh_build_ms = 0.0;
+};
+
+struct HostScene {
+    HostBvh bvh[7];                // sphere, strand, triangle, hf, se, lathe, prism
+    std::vector<float> tri_leaf;   // 12 floats per triangle leaf slot
+    std::vector<HostTexture> tex;
+    HostSceneInfo info;
+};
+
+// Bounds of primitive i of category cat by the reference's rules
+// (bvh.h:24-66, 235-242, 304-331, 394-407).
+void host_prim_bounds(const SheenScene& s, int cat, int i, float* mn, float* mx);
+
+// 2x2 box mip chain with rounding (s+2)/4 (reference scene.h:230-261).
+void host_build_mips(HostTexture& t);
+
+// Validates references and builds every category BVH.  false + err on error.
+bool host_prepare_scene(const SheenScene& s, HostScene& out, std::string& err);
+
+// Linear -> sRGB byte (reference png_color.h:11-21).
+unsigned char host_srgb_byte(float c);
+
+struct SheenContext;
+bool photon_pass(SheenContext* c, const SheenScene* in);
+void sheen_set_error(const char* fmt, ...);

@@ -1,0 +1,622 @@
+// kernels.h — wavefront kernels of one frame chunk.
+//
+// A chunk is a contiguous range of sample slots.  Slot g maps to 8x8 pixel
+// tiles so that one wave64 holds the same sample of 64 neighbouring pixels:
+//   lane = g & 63, s = (g >> 6) % ns, tile = (g >> 6) / ns.
+// Per chunk: primary trace -> primary shading (shadow requests, reflection
+// and first refraction rays) -> up to three rounds of secondary trace and
+// shading -> shadow trace -> per-pixel accumulation in the reference's order
+// (render_kernel.cu:491-813).
+#pragma once
+#include "dev_shade.h"
+
+struct FrameDesc {
+    int width, height, ns;
+    int tiles_x;
+    uint32_t g0;       // first slot of the chunk
+    uint32_t n_slots;  // slots in the chunk (multiple of 64*ns)
+};
+
+// Sample slot -> pixel / sample index.
+SHD bool slot_pixel(const FrameDesc& F, uint32_t g, int& x, int& y, int& s) {
+    uint32_t lane = g & 63u, blk = g >> 6;
+    s = (int)(blk % (uint32_t)F.ns);
+    uint32_t tile = blk / (uint32_t)F.ns;
+    x = (int)((tile % (uint32_t)F.tiles_x) * 8u + (lane & 7u));
+    y = (int)((tile / (uint32_t)F.tiles_x) * 8u + (lane >> 3));
+    return x < F.width && y < F.height;
+}
+
+// Camera ray of one sample (reference render_kernel.cu:491-521 with
+// sampling/aa.h:25-71).
+SHD DRay camera_ray(const CameraData& cam, int x, int y, int s, int ns, int W, int H, float& pix_angle,
+                    uint32_t& seed) {
+    uint32_t js = (uint32_t(x) * 73856093u) ^ (uint32_t(y) * 19349663u) ^ (uint32_t(s) * 83492791u);
+    float ox, oy;
+    if (ns <= 4) {
+        float cx = (s == 0 || s == 2) ? 0.25f : 0.75f;
+        float cy = (s < 2) ? 0.25f : 0.75f;
+        float jx = rand01(js) - 0.5f;
+        float jy = rand01(js ^ 0x9e3779b9u) - 0.5f;
+        ox = cx + jx * 0.5f;
+        oy = cy + jy * 0.5f;
+    } else {
+        int side = 2;
+        while (side * side < ns) side++;
+        int cx = s % side, cy = s / side;
+        float inv = 1.0f / (float)side;
+        ox = ((float)cx + rand01(js)) * inv;
+        oy = ((float)cy + rand01(js ^ 0x9e3779b9u)) * inv;
+    }
+    float u = ((float)x + ox) / (float)W * 2.0f - 1.0f;
+    float v = 1.0f - ((float)y + oy) / (float)H * 2.0f;
+    seed = mix32(js);
+    float tan_h = 1.0f, tan_x = 1.0f, tan_y = 1.0f;
+    if (cam.focal_length > 0.0f) {
+        tan_h = 0.5f * cam.sensor_width / cam.focal_length;
+        if (W >= H) { tan_x = tan_h; tan_y = tan_h * (float)H / (float)W; }
+        else { tan_y = tan_h; tan_x = tan_h * (float)W / (float)H; }
+    }
+    DRay r;
+    r.o = cv3(cam.position);
+    r.d = normalize(cv3(cam.forward) + cv3(cam.right) * (u * tan_x + cam.shift_x * tan_h) +
+                    cv3(cam.up) * (v * tan_y + cam.shift_y * tan_h));
+    r.tmin = 0.001f;
+    r.tmax = 1e10f;
+    pix_angle = 2.0f * tan_x / (float)W;
+    return r;
+}
+
+struct SampleBufs {
+    int4* hit_p;        // [slots] primary hits
+    float4* samp;       // [slots] x: reflect weight, y: transmission, z: chain weight, w: flags
+    float4* beam_add;   // [slots * n_beam] or null
+    float4* media_p;    // [slots] or null: inscatter xyz, exp(-tau) w
+    float4* media_r;    // [slots] or null
+    int n_beam;
+};
+#define SF_VALID 1
+#define SF_HIT 2
+#define SF_REFL 4
+#define SF_REFR 8
+#define SF_EV2 16
+
+// Secondary ray queue entry (reflection or refraction face), 32 bytes.
+struct SecRay {
+    float ox, oy, oz, w;
+    float dx, dy, dz;
+    uint32_t meta;  // bits 0..27 slot, 28 refraction, 29 entering, 30..31 face
+};
+
+struct Counters {
+    uint32_t sec_n[4];       // secondary queues (ping-pong) sizes
+    uint32_t shadow_n;
+    uint32_t pad[3];
+    unsigned long long shadow_passes;
+    unsigned long long nodes, prims, snodes, sprims;
+    unsigned long long reflection, refraction, extra_shadow;
+};
+
+SHD uint32_t wave_sum(uint32_t v) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+SHD void wave_add64(unsigned long long* dst, uint32_t v) {
+    uint32_t t = wave_sum(v);
+    if (__lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1) atomicAdd(dst, (unsigned long long)t);
+}
+
+// Shadow test of one request (reference render_kernel.cu:237-280).
+template <bool COUNT>
+SHD float shadow_eval(const DScene& S, const ShadowReq& q, TStack& st, uint32_t& passes, TraceStats& ts) {
+    DRay r;
+    r.o = mk3(q.ox, q.oy, q.oz);
+    r.d = mk3(q.dx, q.dy, q.dz);
+    r.tmin = 0.001f;
+    r.tmax = q.tmax;
+    const DRay orig = r;
+    bool blocked = false;
+    for (int pass = 0; pass < 3; pass++) {
+        passes++;
+        QHit h;
+        if (!query_opaque<COUNT>(S, r, h, st, ts)) { blocked = false; break; }
+        const Material& m = S.materials[hit_material(S, h, r)];
+        bool through = false;
+        if (m.transmission > 0.5f) {
+            DHit full;
+            rebuild_hit(S, r, h, full);
+            if (transmission_at(S, full.uvx, full.uvy, m) > 0.5f) {
+                r.o = full.point + r.d * 0.002f;
+                through = true;
+            }
+        }
+        if (!through) { blocked = true; break; }
+    }
+    if (blocked) return 0.0f;
+    if (q.slot & 0x80000000u) return expf(-strands_depth(S, orig));
+    return 1.0f;
+}
+
+// Occlusion query used by beams and media (scene_intersect_opaque only).
+SHD bool occluded(const DScene& S, const DRay& r, TStack& st, QHit& h) {
+    TraceStats ts{0, 0};
+    return query_opaque<false>(S, r, h, st, ts);
+}
+
+// Spot-light beams of one primary sample (reference render_kernel.cu:560-652).
+SHD void beams_of(const DScene& S, const DRay& ray, bool hit, float hit_t, uint32_t seed, float4* out, TStack& st,
+                  uint32_t& passes) {
+    int k_out = 0;
+    for (int i = 0; i < S.num_lights; i++) {
+        const Light& light = S.lights[i];
+        if (light.type != 3 || light.beam <= 0.0f) continue;
+        v3 acc = mk3(0, 0, 0);
+        float t_end = hit ? fminf(hit_t, 30.0f) : 30.0f;
+        if (t_end > 0.0f) {
+            const int STEPS = 6;
+            float dt = t_end / (float)STEPS;
+            float cos_half = cosf(light.angle);
+            float blend = fminf(fmaxf(light.blend, 0.0f), 1.0f);
+            float cos_soft = cosf(light.angle * (1.0f - blend));
+            for (int k = 0; k < STEPS; k++) {
+                float t = ((float)k + 0.5f) * dt;
+                v3 p = ray.o + ray.d * t;
+                v3 vv = p - cv3(light.position);
+                float d2 = fmaxf(dot(vv, vv), 1.0e-4f);
+                float c = dot(vv, cv3(light.direction)) * (1.0f / sqrtf(d2));
+                if (c <= cos_half) continue;
+                float w = 1.0f;
+                if (cos_soft > cos_half + 1.0e-4f) w = fminf((c - cos_half) / (cos_soft - cos_half), 1.0f);
+                float dens = light.beam * light.intensity * 1.0e-3f / (1.0f + 0.25f * d2);
+                v3 target = cv3(light.position);
+                if (light.radius > 0.0f) {
+                    v3 axis = normalize(-vv);
+                    v3 up = (fabsf(axis.y) < 0.9f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
+                    v3 t1 = normalize(cross(axis, up));
+                    v3 t2 = cross(axis, t1);
+                    float rr = sqrtf(rand01(seed ^ (uint32_t)(uint32_t(k) * 2654435761u) ^ 0x85ebca6bu)) * light.radius;
+                    float ph = 6.2831853f * rand01(seed ^ (uint32_t)(uint32_t(k) * 2246822519u) ^ 0xc2b2ae35u);
+                    target = cv3(light.position) + t1 * (cosf(ph) * rr) + t2 * (sinf(ph) * rr);
+                }
+                v3 to_l = target - p;
+                float ldist = sqrtf(fmaxf(dot(to_l, to_l), 1.0e-8f));
+                v3 ldir = to_l / ldist;
+                float remain = ldist * 0.999f;
+                DRay sr;
+                sr.o = p; sr.d = ldir; sr.tmin = 0.002f; sr.tmax = remain;
+                bool blocked = false;
+                for (int pass = 0; pass < 3; pass++) {
+                    QHit h;
+                    passes++;
+                    if (!occluded(S, sr, st, h)) { blocked = false; break; }
+                    DHit full;
+                    const Material& m = S.materials[hit_material(S, h, sr)];
+                    if (m.transmission > 0.5f) {
+                        rebuild_hit(S, sr, h, full);
+                        remain -= full.t + 0.002f;
+                        if (remain <= 0.0f) { blocked = false; break; }
+                        sr.o = full.point + ldir * 0.002f;
+                        sr.tmax = remain;
+                        continue;
+                    }
+                    blocked = true;
+                    break;
+                }
+                if (blocked) continue;
+                acc = acc + cv3(light.color) * (dens * w * dt);
+            }
+        }
+        out[k_out++] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+    }
+}
+
+SHD bool ray_box(const DRay& r, v3 c, v3 h, float& t0, float& t1) {  // render_kernel.cu:302-325
+    t0 = -1.0e30f; t1 = 1.0e30f;
+    const float o[3] = {r.o.x, r.o.y, r.o.z}, d[3] = {r.d.x, r.d.y, r.d.z};
+    const float cc[3] = {c.x, c.y, c.z}, hh[3] = {h.x, h.y, h.z};
+    for (int a = 0; a < 3; a++) {
+        if (fabsf(d[a]) < 1.0e-8f) {
+            if (o[a] < cc[a] - hh[a] || o[a] > cc[a] + hh[a]) return false;
+            continue;
+        }
+        float inv = 1.0f / d[a];
+        float tn = (cc[a] - hh[a] - o[a]) * inv;
+        float tf = (cc[a] + hh[a] - o[a]) * inv;
+        if (tn > tf) { float t = tn; tn = tf; tf = t; }
+        t0 = fmaxf(t0, tn);
+        t1 = fminf(t1, tf);
+        if (t0 > t1) return false;
+    }
+    return true;
+}
+
+// Single-scatter media along a ray (reference render_kernel.cu:339-436).
+SHD float4 media_of(const DScene& S, const DRay& ray, float t_surface, uint32_t seed, TStack& st, uint32_t& passes) {
+    v3 in_scatter = mk3(0, 0, 0);
+    float tau_total = 0.0f;
+    for (int m = 0; m < S.num_media; m++) {
+        const Medium& med = S.media[m];
+        if (med.samples <= 0) continue;
+        float m0, m1;
+        if (!ray_box(ray, cv3(med.center), cv3(med.half), m0, m1)) continue;
+        m0 = fmaxf(m0, ray.tmin);
+        m1 = fminf(m1, t_surface);
+        if (m1 <= m0) continue;
+        int n = med.samples > 64 ? 64 : med.samples;
+        float dt = (m1 - m0) / (float)n;
+        float tau = 0.0f;
+        for (int k = 0; k < n; k++) {
+            float jitter = rand01(seed ^ (uint32_t)(uint32_t(m) * 9781u) ^ (uint32_t)(uint32_t(k) * 6271u));
+            float t = m0 + ((float)k + jitter) * dt;
+            v3 p = ray.o + ray.d * t;
+            v3 radiance = mk3(0, 0, 0);
+            for (int i = 0; i < S.num_lights; i++) {
+                const Light& light = S.lights[i];
+                if (!(light.flags & 1)) continue;
+                v3 ldir, lrad;
+                float ldist;
+                if (light.type == 0) {
+                    ldir = sun_dir(light, seed ^ (uint32_t)(uint32_t(i) * 2654435761u) ^ (uint32_t)(uint32_t(k) * 2246822519u));
+                    ldist = 1.0e9f;
+                    lrad = cv3(light.color) * light.intensity;
+                } else {
+                    v3 tl = cv3(light.position) - p;
+                    float d2 = fmaxf(dot(tl, tl), 1.0e-4f);
+                    ldist = sqrtf(d2);
+                    ldir = tl / ldist;
+                    lrad = cv3(light.color) * (light.intensity / (12.5663706f * d2));
+                    if (light.type == 3) {
+                        float c = dot(-ldir, cv3(light.direction));
+                        float cos_half = cosf(light.angle);
+                        if (c <= cos_half) continue;
+                        float blend = fminf(fmaxf(light.blend, 0.0f), 1.0f);
+                        float cos_soft = cosf(light.angle * (1.0f - blend));
+                        if (cos_soft > cos_half + 1.0e-4f) lrad = lrad * fminf((c - cos_half) / (cos_soft - cos_half), 1.0f);
+                    }
+                }
+                DRay sr;
+                sr.o = p; sr.d = ldir; sr.tmin = 0.002f; sr.tmax = ldist;
+                QHit h;
+                passes++;
+                if (occluded(S, sr, st, h)) continue;
+                radiance = radiance + lrad;
+            }
+            v3 scatter = cv3(med.sigma_s) * radiance * 0.07957747f;
+            in_scatter = in_scatter + scatter * (expf(-tau) * dt);
+            tau += med.sigma_t * dt;
+        }
+        tau_total += tau;
+    }
+    return make_float4(in_scatter.x, in_scatter.y, in_scatter.z, expf(-tau_total));
+}
+
+// First/next refraction face (reference render_kernel.cu:750-765).
+SHD void refract_from(const DScene& S, const DHit& gh, v3 d, bool entering, float& w, v3& o_out, v3& td) {
+    const Material& sm = S.materials[gh.material_id];
+    v3 n = gh.normal;
+    float ior = fmaxf(ior_at(S, gh.uvx, gh.uvy, sm), 1.0f);
+    float eta = entering ? (1.0f / ior) : ior;
+    float cosi = fmaxf(-dot(d, n), 1.0e-4f);
+    float k = 1.0f - eta * eta * (1.0f - cosi * cosi);
+    td = (k < 0.0f) ? normalize(d + 2.0f * cosi * n) : normalize(eta * d + (eta * cosi - sqrtf(k)) * n);
+    w *= transmission_at(S, gh.uvx, gh.uvy, sm);
+    o_out = gh.point + td * 0.002f;
+}
+
+SHD TStack make_stack(uint2* shared) {
+    TStack st;
+    st.lds = shared + threadIdx.x;
+    st.sp = 0;
+    return st;
+}
+
+// ---------------------------------------------------------------------------
+template <bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_primary(DScene S, FrameDesc F, int4* hit_p, Counters* C) {
+    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];
+    TStack st = make_stack(s_stack);
+    TraceStats ts{0, 0};
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < F.n_slots; i += stride) {
+        int x, y, s;
+        int4 out = make_int4(0, -1, 0, 0);
+        if (slot_pixel(F, F.g0 + i, x, y, s)) {
+            float pa;
+            uint32_t seed;
+            DRay r = camera_ray(S.cam, x, y, s, F.ns, F.width, F.height, pa, seed);
+            QHit q;
+            query_closest<COUNT>(S, r, q, st, ts);
+            out = make_int4(__float_as_int(q.t), q.kind < 0 ? -1 : ((q.kind << 28) | q.idx), q.carry, 0);
+        }
+        hit_p[i] = out;
+    }
+    if (COUNT) {
+        wave_add64(&C->nodes, ts.nodes);
+        wave_add64(&C->prims, ts.prims);
+    }
+}
+
+SHD QHit unpack_hit(int4 h) {
+    QHit q;
+    q.t = __int_as_float(h.x);
+    q.kind = h.y < 0 ? -1 : (int)((uint32_t)h.y >> 28);
+    q.idx = h.y < 0 ? -1 : (h.y & 0x0FFFFFFF);
+    q.carry = h.z;
+    return q;
+}
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shade_primary(DScene S, FrameDesc F, SampleBufs SB, EventBufs EB,
+                                                               SecRay* sec_q, Counters* C) {
+    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];
+    TStack st = make_stack(s_stack);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t passes = 0;
+    // loop bound rounded up to whole waves so every lane reaches the queue ballots
+    const uint32_t n_iter = (F.n_slots + stride - 1) / stride;
+    for (uint32_t it = 0; it < n_iter; it++) {
+        const uint32_t i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        int x = 0, y = 0, s = 0;
+        const bool valid = i < F.n_slots && slot_pixel(F, F.g0 + i, x, y, s);
+        float pa = 0.0f;
+        uint32_t seed = 0;
+        DRay r;
+        QHit q;
+        q.kind = -1;
+        if (valid) {
+            r = camera_ray(S.cam, x, y, s, F.ns, F.width, F.height, pa, seed);
+            q = unpack_hit(SB.hit_p[i]);
+        }
+        const bool hit = valid && q.kind >= 0;
+        DHit rec;
+        if (hit) rebuild_hit(S, r, q, rec);
+        if (valid && !hit) sky_event(S, r.d, i * 3 + 0, EB);
+        shade_event(S, hit, rec, -r.d, hit ? rec.t * pa : 0.0f, seed, i * 3 + 0, EB);
+        float4 sp = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
+        int flags = valid ? SF_VALID : 0;
+        bool want_r = false, want_t = false;
+        SecRay rr, tr;
+        if (hit) {
+            flags |= SF_HIT;
+            const Material& pm = S.materials[rec.material_id];
+            if (pm.reflect > 0.0f) {  // render_kernel.cu:672-680
+                want_r = true;
+                flags |= SF_REFL;
+                sp.x = fminf(pm.reflect, 1.0f);
+                v3 rd = r.d - rec.normal * (2.0f * dot(r.d, rec.normal));
+                v3 o = rec.point + rec.normal * 0.002f;
+                v3 d = normalize(rd);
+                rr.ox = o.x; rr.oy = o.y; rr.oz = o.z; rr.w = 0.0f;
+                rr.dx = d.x; rr.dy = d.y; rr.dz = d.z;
+                rr.meta = i;
+            }
+            float gtr = transmission_at(S, rec.uvx, rec.uvy, pm);  // render_kernel.cu:739
+            if (gtr > 0.0f) {
+                want_t = true;
+                flags |= SF_REFR;
+                sp.y = gtr;
+                float w = 1.0f;
+                v3 o, td;
+                refract_from(S, rec, r.d, true, w, o, td);
+                tr.ox = o.x; tr.oy = o.y; tr.oz = o.z; tr.w = w;
+                tr.dx = td.x; tr.dy = td.y; tr.dz = td.z;
+                tr.meta = i | (1u << 28) | (1u << 29);
+            }
+        }
+        if (valid && SB.beam_add) beams_of(S, r, hit, hit ? rec.t : 0.0f, seed, SB.beam_add + (size_t)i * SB.n_beam, st, passes);
+        if (valid && SB.media_p) SB.media_p[i] = media_of(S, r, hit ? rec.t : 1.0e9f, seed, st, passes);
+        uint32_t a = wave_reserve(&C->sec_n[0], want_r);
+        if (want_r) sec_q[a] = rr;
+        uint32_t b = wave_reserve(&C->sec_n[0], want_t);
+        if (want_t) sec_q[b] = tr;
+        if (i < F.n_slots) {
+            sp.w = __int_as_float(flags);
+            SB.samp[i] = sp;
+        }
+    }
+    wave_add64(&C->extra_shadow, passes);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_sec(DScene S, const SecRay* q_in, int which, int4* hits,
+                                                          Counters* C) {
+    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];
+    TStack st = make_stack(s_stack);
+    TraceStats ts{0, 0};
+    const uint32_t n = C->sec_n[which];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t nrefl = 0, nrefr = 0;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        SecRay e = q_in[j];
+        DRay r;
+        r.o = mk3(e.ox, e.oy, e.oz);
+        r.d = mk3(e.dx, e.dy, e.dz);
+        r.tmin = 0.001f;
+        r.tmax = 1.0e10f;
+        QHit q;
+        query_closest<COUNT>(S, r, q, st, ts);
+        hits[j] = make_int4(__float_as_int(q.t), q.kind < 0 ? -1 : ((q.kind << 28) | q.idx), q.carry, 0);
+        if (e.meta & (1u << 28)) nrefr++; else nrefl++;
+    }
+    wave_add64(&C->reflection, nrefl);
+    wave_add64(&C->refraction, nrefr);
+    if (COUNT) {
+        wave_add64(&C->nodes, ts.nodes);
+        wave_add64(&C->prims, ts.prims);
+    }
+}
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shade_sec(DScene S, FrameDesc F, SampleBufs SB, EventBufs EB,
+                                                          const SecRay* q_in, int which, const int4* hits,
+                                                          SecRay* q_out, int which_out, Counters* C) {
+    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];
+    TStack st = make_stack(s_stack);
+    const uint32_t n = C->sec_n[which];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t n_iter = (n + stride - 1) / stride;
+    uint32_t passes = 0;
+    float tan_x = 1.0f;
+    if (S.cam.focal_length > 0.0f) {
+        float tan_h = 0.5f * S.cam.sensor_width / S.cam.focal_length;
+        tan_x = (F.width >= F.height) ? tan_h : tan_h * (float)F.width / (float)F.height;
+    }
+    const float pa = 2.0f * tan_x / (float)F.width;
+    for (uint32_t it = 0; it < n_iter; it++) {
+        const uint32_t j = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        const bool valid = j < n;
+        SecRay e;
+        QHit q;
+        q.kind = -1;
+        uint32_t i = 0;
+        bool refr = false;
+        DRay r;
+        if (valid) {
+            e = q_in[j];
+            q = unpack_hit(hits[j]);
+            i = e.meta & 0x0FFFFFFFu;
+            refr = (e.meta >> 28) & 1u;
+            r.o = mk3(e.ox, e.oy, e.oz);
+            r.d = mk3(e.dx, e.dy, e.dz);
+            r.tmin = 0.001f;
+            r.tmax = 1.0e10f;
+        }
+        int x = 0, y = 0, s = 0;
+        uint32_t seed = 0;
+        if (valid) {
+            slot_pixel(F, F.g0 + i, x, y, s);
+            seed = mix32((uint32_t(x) * 73856093u) ^ (uint32_t(y) * 19349663u) ^ (uint32_t(s) * 83492791u));
+        }
+        const bool hit = valid && q.kind >= 0;
+        DHit h;
+        if (hit) rebuild_hit(S, r, q, h);
+        bool lit = false;
+        uint32_t ev = 0;
+        bool want_next = false;
+        SecRay nx;
+        if (valid && !refr) {  // reflection: render_kernel.cu:682-711
+            ev = i * 3 + 1;
+            if (hit) lit = true;
+            else sky_event(S, r.d, ev, EB);
+            if (SB.media_r) SB.media_r[i] = media_of(S, r, hit ? h.t : 1.0e9f, seed, st, passes);
+        } else if (valid) {  // refraction face: render_kernel.cu:767-802
+            ev = i * 3 + 2;
+            const uint32_t face = e.meta >> 30;
+            const bool entering = (e.meta >> 29) & 1u;
+            float w = e.w;
+            if (!hit) {
+                sky_event(S, r.d, ev, EB);
+                float4 sp = SB.samp[i];
+                sp.z = w;
+                sp.w = __int_as_float(__float_as_int(sp.w) | SF_EV2);
+                SB.samp[i] = sp;
+            } else if (transmission_at(S, h.uvx, h.uvy, S.materials[h.material_id]) > 0.0f) {
+                if (face < 2) {
+                    v3 o, td;
+                    refract_from(S, h, r.d, !entering, w, o, td);
+                    nx.ox = o.x; nx.oy = o.y; nx.oz = o.z; nx.w = w;
+                    nx.dx = td.x; nx.dy = td.y; nx.dz = td.z;
+                    nx.meta = i | (1u << 28) | ((!entering) ? (1u << 29) : 0u) | ((face + 1) << 30);
+                    want_next = true;
+                }
+            } else {
+                lit = true;
+                float4 sp = SB.samp[i];
+                sp.z = w;
+                sp.w = __int_as_float(__float_as_int(sp.w) | SF_EV2);
+                SB.samp[i] = sp;
+            }
+        }
+        shade_event(S, lit, h, -r.d, lit ? h.t * pa : 0.0f, seed, ev, EB);
+        uint32_t a = wave_reserve(&C->sec_n[which_out], want_next);
+        if (want_next) q_out[a] = nx;
+    }
+    wave_add64(&C->extra_shadow, passes);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DScene S, EventBufs EB, Counters* C) {
+    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];
+    TStack st = make_stack(s_stack);
+    TraceStats ts{0, 0};
+    const uint32_t n = C->shadow_n;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t passes = 0;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        ShadowReq q = EB.shadow_q[j];
+        float tr = shadow_eval<COUNT>(S, q, st, passes, ts);
+        EB.trans[q.slot & 0x7FFFFFFFu] = tr;
+    }
+    wave_add64(&C->shadow_passes, passes);
+    if (COUNT) {
+        wave_add64(&C->snodes, ts.nodes);
+        wave_add64(&C->sprims, ts.prims);
+    }
+}
+
+// Per-pixel accumulation (reference render_kernel.cu:536-813 blend order).
+__global__ __launch_bounds__(256) void k_accumulate(DScene S, FrameDesc F, SampleBufs SB, EventBufs EB,
+                                                   float4* frame) {
+    const uint32_t n_pix = F.n_slots / (uint32_t)F.ns;  // tiles*64 pixels of the chunk
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pix; p += stride) {
+        const uint32_t lane = p & 63u, tile_local = p >> 6;
+        int x = 0, y = 0, s0 = 0;
+        const uint32_t i0 = tile_local * 64u * (uint32_t)F.ns + lane;
+        if (!slot_pixel(F, F.g0 + i0, x, y, s0)) continue;
+        v3 acc = mk3(0, 0, 0);
+        for (int s = 0; s < F.ns; s++) {
+            const uint32_t i = i0 + (uint32_t)s * 64u;
+            const float4 sp = SB.samp[i];
+            const int flags = __float_as_int(sp.w);
+            v3 c = event_value(S, i * 3 + 0, EB);
+            for (int k = 0; k < SB.n_beam && SB.beam_add; k++) {
+                float4 b = SB.beam_add[(size_t)i * SB.n_beam + k];
+                c = c + mk3(b.x, b.y, b.z);
+            }
+            if (SB.media_p) {
+                float4 m = SB.media_p[i];
+                c = c * m.w + mk3(m.x, m.y, m.z);
+            }
+            if (flags & SF_REFL) {
+                v3 rc = event_value(S, i * 3 + 1, EB);
+                if (SB.media_r) {
+                    float4 m = SB.media_r[i];
+                    rc = rc * m.w + mk3(m.x, m.y, m.z);
+                }
+                const float w = sp.x;
+                c = c * (1.0f - w) + rc * w;
+            }
+            if (flags & SF_REFR) {
+                v3 through = mk3(0, 0, 0);
+                if (flags & SF_EV2) through = through + sp.z * event_value(S, i * 3 + 2, EB);
+                c = c * (1.0f - fminf(sp.y, 1.0f)) + through;
+            }
+            acc = acc + c;
+        }
+        v3 f = acc * (1.0f / (float)F.ns);
+        frame[(size_t)y * F.width + x] = make_float4(f.x, f.y, f.z, 1.0f);
+    }
+}
+
+__global__ void k_reset(Counters* C) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        for (int k = 0; k < 4; k++) C->sec_n[k] = 0;
+        C->shadow_n = 0;
+    }
+}
+__global__ void k_clear_sec(Counters* C, int which) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) C->sec_n[which] = 0;
+}
+
+// Primary hits of a frame chunk for parity checks: kind/index/t per sample.
+__global__ void k_export_hits(FrameDesc F, const int4* hit_p, int* kind, int* index, float* t) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < F.n_slots; i += stride) {
+        int x, y, s;
+        if (!slot_pixel(F, F.g0 + i, x, y, s)) continue;
+        const size_t o = ((size_t)y * F.width + x) * F.ns + s;
+        const int4 h = hit_p[i];
+        kind[o] = h.y < 0 ? -1 : (int)((uint32_t)h.y >> 28);
+        index[o] = h.y < 0 ? -1 : (h.y & 0x0FFFFFFF);
+        t[o] = h.y < 0 ? 0.0f : __int_as_float(h.x);
+    }
+}

@@ -1,0 +1,610 @@
+// render.hip — host side of the HIP render path and the C ABI
+// (include/sheen_c_api.h).  One translation unit holds the kernels
+// (kernels.h) so the device code is compiled once for gfx950.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include <hipcub/hipcub.hpp>
+
+#include "dev_photon.h"
+#include "kernels.h"
+#include "sheen_c_api.h"
+#include "host_util.h"
+
+// ---------------------------------------------------------------------------
+// Error state (reference api/sheen_c_api.cu:21-26: static 512-byte buffer)
+// ---------------------------------------------------------------------------
+static char g_error[512] = "no error";
+void sheen_set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_error, sizeof(g_error), fmt, ap);
+    va_end(ap);
+}
+
+#define HIP_OR(expr, ret)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            sheen_set_error("%s failed: %s", #expr, hipGetErrorString(e_));                \
+            return ret;                                                                    \
+        }                                                                                  \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct WorkBufs {
+    size_t cap_slots = 0;
+    int cap_L = -1, cap_beam = -1;
+    bool cap_caustic = false, cap_media = false;
+    int4* hit_p = nullptr;
+    float4* samp = nullptr;
+    float4* ev_base = nullptr;
+    float4* ev_caustic = nullptr;
+    float4* contrib = nullptr;
+    float* trans = nullptr;
+    ShadowReq* shadow_q = nullptr;
+    SecRay* sec_q[2] = {nullptr, nullptr};
+    int4* sec_hits = nullptr;
+    float4* beam_add = nullptr;
+    float4* media_p = nullptr;
+    float4* media_r = nullptr;
+    void release() {
+        void* ps[] = {hit_p, samp, ev_base, ev_caustic, contrib, trans, shadow_q, sec_q[0], sec_q[1], sec_hits,
+                      beam_add, media_p, media_r};
+        for (void* p : ps) if (p) (void)hipFree(p);
+        *this = WorkBufs();
+    }
+};
+
+struct SheenContext {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    std::vector<void*> owned;
+    DScene ds{};
+    HostSceneInfo info;
+    int n_beam = 0;
+    WorkBufs wb;
+    float4* frame = nullptr;
+    size_t frame_cap = 0;
+    int frame_w = 0, frame_h = 0;
+    Counters* ctr = nullptr;
+    bool profiling = false, counting = false;
+    long long chunk_override = 0;
+    hipEvent_t ev_frame[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> ev_pool;
+    SheenRayStats last{};
+    Photon* photons = nullptr;  // sorted by cell
+    int* cell_start = nullptr;
+    int* cell_count = nullptr;
+    int n_photons = 0;
+
+    ~SheenContext() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        wb.release();
+        if (frame) (void)hipFree(frame);
+        if (ctr) (void)hipFree(ctr);
+        if (photons) (void)hipFree(photons);
+        if (cell_start) (void)hipFree(cell_start);
+        if (cell_count) (void)hipFree(cell_count);
+        for (void* p : owned) (void)hipFree(p);
+        for (hipEvent_t e : ev_frame) if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    template <class T>
+    T* upload(const T* src, size_t n) {
+        if (!src || n == 0) return nullptr;
+        void* p = nullptr;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+        owned.push_back(p);
+        if (hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        return (T*)p;
+    }
+};
+
+static int grid_for(SheenContext* c, size_t n, int per_cu) {
+    size_t blocks = (n + TRACE_BLOCK - 1) / TRACE_BLOCK;
+    size_t cap = (size_t)c->num_cu * per_cu;
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    return (int)blocks;
+}
+
+static bool alloc_dev(void** p, size_t bytes) {
+    *p = nullptr;
+    return hipMalloc(p, bytes ? bytes : 16) == hipSuccess;
+}
+
+// Photon pass and deterministic hash-grid map (reference
+// api/sheen_c_api.cu:271-307, render_kernel.cu:928-1045).
+bool photon_pass(SheenContext* c, const SheenScene* in) {
+    const long long total_ll = (long long)in->photons * c->ds.num_lights;
+    if (total_ll <= 0) return true;
+    if (total_ll > 0x7FFFFFFF) { sheen_set_error("too many photons"); return false; }
+    const uint32_t total = (uint32_t)total_ll;
+    const uint32_t cap = (uint32_t)std::min<long long>(total_ll, 4000000);  // sheen_c_api.cu:276-277
+    const float cell = in->photon_cell > 0.0f ? in->photon_cell : 0.25f;    // sheen_c_api.cu:306
+    const int num_cells = 1 << 18;                                           // render_kernel.cu:997
+    hipStream_t s = c->stream;
+    Photon *all = nullptr, *stored = nullptr;
+    uint32_t *flag = nullptr, *pos = nullptr, *keys = nullptr, *keys2 = nullptr, *vals = nullptr, *vals2 = nullptr;
+    PhotonCounters* pc = nullptr;
+    void* tmp = nullptr;
+    bool ok = false;
+    do {
+        if (!alloc_dev((void**)&all, (size_t)total * sizeof(Photon)) || !alloc_dev((void**)&flag, (size_t)total * 4) ||
+            !alloc_dev((void**)&pos, (size_t)total * 4) || !alloc_dev((void**)&pc, sizeof(PhotonCounters))) {
+            sheen_set_error("photon alloc failed");
+            break;
+        }
+        PhotonCounters zero{};
+        if (hipMemcpyAsync(pc, &zero, sizeof(zero), hipMemcpyHostToDevice, s) != hipSuccess) break;
+        k_photon_emit<<<grid_for(c, total, 16), TRACE_BLOCK, 0, s>>>(c->ds, in->photons, total, all, flag, pc);
+        size_t tb = 0;
+        if (hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag, pos, (int)total, s) != hipSuccess) break;
+        if (!alloc_dev(&tmp, tb)) { sheen_set_error("photon scan alloc failed"); break; }
+        if (hipcub::DeviceScan::ExclusiveSum(tmp, tb, flag, pos, (int)total, s) != hipSuccess) break;
+        uint32_t last[2] = {0, 0};
+        PhotonCounters hpc{};
+        if (hipMemcpyAsync(&last[0], pos + total - 1, 4, hipMemcpyDeviceToHost, s) != hipSuccess) break;
+        if (hipMemcpyAsync(&last[1], flag + total - 1, 4, hipMemcpyDeviceToHost, s) != hipSuccess) break;
+        if (hipMemcpyAsync(&hpc, pc, sizeof(hpc), hipMemcpyDeviceToHost, s) != hipSuccess) break;
+        if (hipStreamSynchronize(s) != hipSuccess) break;
+        c->info.photons_emitted = (int)total;
+        c->info.photon_rays = hpc.rays;
+        const uint32_t n = std::min(last[0] + last[1], cap);
+        c->info.photons_stored = (int)n;
+        c->n_photons = (int)n;
+        if (n == 0) { ok = true; break; }
+        if (!alloc_dev((void**)&stored, (size_t)n * sizeof(Photon)) || !alloc_dev((void**)&keys, (size_t)n * 4) ||
+            !alloc_dev((void**)&keys2, (size_t)n * 4) || !alloc_dev((void**)&vals, (size_t)n * 4) ||
+            !alloc_dev((void**)&vals2, (size_t)n * 4) || !alloc_dev((void**)&c->photons, (size_t)n * sizeof(Photon)) ||
+            !alloc_dev((void**)&c->cell_count, (size_t)num_cells * 4) ||
+            !alloc_dev((void**)&c->cell_start, (size_t)num_cells * 4)) {
+            sheen_set_error("photon map alloc failed");
+            break;
+        }
+        k_photon_compact<<<grid_for(c, total, 8), 256, 0, s>>>(all, flag, pos, total, cap, stored);
+        k_zero_i32<<<grid_for(c, num_cells, 8), 256, 0, s>>>(c->cell_count, num_cells);
+        k_photon_keys<<<grid_for(c, n, 8), 256, 0, s>>>(stored, n, cell, num_cells, keys, vals, c->cell_count);
+        (void)hipFree(tmp);
+        tmp = nullptr;
+        size_t sb = 0;
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, sb, keys, keys2, vals, vals2, (int)n, 0, 18, s) != hipSuccess) break;
+        size_t sb2 = 0;
+        if (hipcub::DeviceScan::ExclusiveSum(nullptr, sb2, c->cell_count, c->cell_start, num_cells, s) != hipSuccess) break;
+        if (!alloc_dev(&tmp, std::max(sb, sb2))) { sheen_set_error("photon sort alloc failed"); break; }
+        if (hipcub::DeviceRadixSort::SortPairs(tmp, sb, keys, keys2, vals, vals2, (int)n, 0, 18, s) != hipSuccess) break;
+        if (hipcub::DeviceScan::ExclusiveSum(tmp, sb2, c->cell_count, c->cell_start, num_cells, s) != hipSuccess) break;
+        k_photon_gather_sorted<<<grid_for(c, n, 8), 256, 0, s>>>(stored, vals2, n, c->photons);
+        if (hipStreamSynchronize(s) != hipSuccess) break;
+        c->ds.pm.photons = c->photons;
+        c->ds.pm.cell_start = c->cell_start;
+        c->ds.pm.cell_count = c->cell_count;
+        c->ds.pm.num_cells = num_cells;
+        c->ds.pm.cell = cell;
+        ok = true;
+    } while (false);
+    if (!ok && std::strcmp(g_error, "no error") == 0) sheen_set_error("photon pass failed: %s", hipGetErrorString(hipGetLastError()));
+    void* fr[] = {all, stored, flag, pos, keys, keys2, vals, vals2, pc, tmp};
+    for (void* p : fr) if (p) (void)hipFree(p);
+    return ok;
+}
+
+// Uploads a scene: BVHs, records, textures.  Returns nullptr on error.
+static SheenContext* context_create(const SheenScene* in) {
+    if (!in) { sheen_set_error("null scene"); return nullptr; }
+    if (in->num_materials <= 0 || !in->materials) { sheen_set_error("no materials"); return nullptr; }
+    std::unique_ptr<SheenContext> c(new SheenContext());
+    HIP_OR(hipGetDevice(&c->device), nullptr);
+    hipDeviceProp_t prop;
+    HIP_OR(hipGetDeviceProperties(&prop, c->device), nullptr);
+    c->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    HIP_OR(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), nullptr);
+    HIP_OR(hipEventCreate(&c->ev_frame[0]), nullptr);
+    HIP_OR(hipEventCreate(&c->ev_frame[1]), nullptr);
+
+    HostScene hs;
+    std::string err;
+    if (!host_prepare_scene(*in, hs, err)) { sheen_set_error("%s", err.c_str()); return nullptr; }
+    c->info = hs.info;
+    DScene& d = c->ds;
+    std::memset(&d, 0, sizeof(d));
+    for (int k = 0; k < N_BVH_CATS; k++) {
+        const HostBvh& hb = hs.bvh[k];
+        d.count[k] = hb.nodes.empty() ? 0 : hb.count;
+        d.cat_opaque[k] = hb.opaque;
+        if (hb.nodes.empty()) continue;
+        d.bvh[k] = c->upload(hb.nodes.data(), hb.nodes.size());
+        d.prim_ids[k] = c->upload(hb.prims.data(), hb.prims.size());
+        d.bounds[k] = c->upload(hb.bounds.data(), hb.bounds.size());
+        if (!d.bvh[k] || !d.prim_ids[k]) { sheen_set_error("BVH upload failed"); return nullptr; }
+    }
+    if (!hs.tri_leaf.empty()) {
+        d.tri_leaf = c->upload((const float4*)hs.tri_leaf.data(), hs.tri_leaf.size() / 4);
+        if (!d.tri_leaf) { sheen_set_error("triangle upload failed"); return nullptr; }
+    }
+    d.objects = c->upload(in->objects, in->objects ? (size_t)in->num_objects : 0);
+    d.strands = c->upload(in->sweep_strands, in->sweep_strands ? (size_t)in->num_sweep_strands : 0);
+    d.sweep_points = c->upload(in->sweep_points, in->sweep_points ? (size_t)in->num_sweep_points : 0);
+    d.triangles = c->upload(in->triangles, in->triangles ? (size_t)in->num_triangles : 0);
+    d.num_triangles = in->triangles ? in->num_triangles : 0;
+    // height fields: image pointers become device pointers (reference api/sheen_c_api.cu:220-236)
+    if (in->heightfields && in->num_heightfields > 0) {
+        std::vector<HeightField> hf(in->heightfields, in->heightfields + in->num_heightfields);
+        for (auto& h : hf) {
+            if (h.image && h.img_w > 0 && h.img_h > 0) {
+                h.image = c->upload(h.image, (size_t)h.img_w * h.img_h);
+                if (!h.image) { sheen_set_error("hf image alloc failed"); return nullptr; }
+            } else {
+                h.image = nullptr;
+            }
+        }
+        d.hfs = c->upload(hf.data(), hf.size());
+    }
+    d.ses = c->upload(in->superellipsoids, in->superellipsoids ? (size_t)in->num_superellipsoids : 0);
+    d.lathes = c->upload(in->lathes, in->lathes ? (size_t)in->num_lathes : 0);
+    d.lathe_points = c->upload(in->lathe_points, in->lathe_points ? (size_t)in->num_lathe_points : 0);
+    d.prisms = c->upload(in->prisms, in->prisms ? (size_t)in->num_prisms : 0);
+    d.prism_points = c->upload(in->prism_points, in->prism_points ? (size_t)in->num_prism_points : 0);
+    d.csg = c->upload(in->csg_nodes, in->csg_nodes ? (size_t)in->num_csg_nodes : 0);
+    d.num_csg = in->csg_nodes ? in->num_csg_nodes : 0;
+    d.csg_root = (d.csg && in->csg_root >= 0 && in->csg_root < d.num_csg) ? in->csg_root : -1;
+    d.planes = c->upload(in->planes, in->planes ? (size_t)in->num_planes : 0);
+    d.num_planes = in->planes ? in->num_planes : 0;
+    d.materials = c->upload(in->materials, (size_t)in->num_materials);
+    d.num_materials = in->num_materials;
+    d.lights = c->upload(in->lights, in->lights ? (size_t)in->num_lights : 0);
+    d.num_lights = in->lights ? in->num_lights : 0;
+    d.media = c->upload(in->media, in->media ? (size_t)in->num_media : 0);
+    d.num_media = in->media ? in->num_media : 0;
+    // textures with their mip chains (reference scene.h:230-261)
+    if (in->num_textures > 0) {
+        std::vector<TextureRef> refs(in->num_textures);
+        std::memset(refs.data(), 0, refs.size() * sizeof(TextureRef));
+        for (int i = 0; i < in->num_textures; i++) {
+            const HostTexture& t = hs.tex[i];
+            if (t.px.empty()) continue;
+            refs[i].pixels = c->upload(t.px.data(), t.px.size());
+            if (!refs[i].pixels) { sheen_set_error("tex alloc failed"); return nullptr; }
+            refs[i].width = t.w;
+            refs[i].height = t.h;
+            refs[i].mips = t.mips;
+        }
+        d.textures = c->upload(refs.data(), refs.size());
+        d.num_textures = in->num_textures;
+    }
+    d.cam = in->camera;
+    d.caustics = in->caustics;
+    for (int i = 0; i < d.num_lights; i++)
+        if (in->lights[i].type == 3 && in->lights[i].beam > 0.0f) c->n_beam++;
+    d.any_beams = c->n_beam > 0;
+    d.any_strands = d.count[C_STRAND] > 0;
+    if (!alloc_dev((void**)&c->ctr, sizeof(Counters))) { sheen_set_error("counter alloc failed"); return nullptr; }
+    if (in->photons > 0 && d.num_lights > 0) {
+        if (!photon_pass(c.get(), in)) return nullptr;
+    }
+    HIP_OR(hipDeviceSynchronize(), nullptr);
+    return c.release();
+}
+
+// Per-slot bytes of the work buffers.
+static size_t slot_bytes(int L, int n_beam, bool caustic, bool media) {
+    size_t b = 16 /*hit*/ + 16 /*samp*/ + 3 * 16 /*base*/ + (caustic ? 3 * 16 : 0) + 3 * (size_t)L * (16 + 4) +
+               2 * 32 /*sec queues*/ + 2 * 16 /*sec hits*/ + 3 * (size_t)L * 32 /*shadow queue*/ +
+               (size_t)n_beam * 16 + (media ? 32 : 0);
+    return b;
+}
+
+static bool ensure_work(SheenContext* c, size_t slots) {
+    WorkBufs& w = c->wb;
+    const int L = c->ds.num_lights;
+    const bool caustic = c->ds.pm.photons && c->ds.caustics > 0.0f;
+    const bool media = c->ds.num_media > 0;
+    if (w.cap_slots >= slots && w.cap_L == L && w.cap_beam == c->n_beam && w.cap_caustic == caustic &&
+        w.cap_media == media)
+        return true;
+    w.release();
+    const size_t ev = slots * 3;
+    const size_t Lq = (size_t)std::max(L, 1);
+    bool ok = alloc_dev((void**)&w.hit_p, slots * sizeof(int4)) && alloc_dev((void**)&w.samp, slots * sizeof(float4)) &&
+              alloc_dev((void**)&w.ev_base, ev * sizeof(float4)) &&
+              alloc_dev((void**)&w.contrib, ev * Lq * sizeof(float4)) &&
+              alloc_dev((void**)&w.trans, ev * Lq * sizeof(float)) &&
+              alloc_dev((void**)&w.shadow_q, ev * Lq * sizeof(ShadowReq)) &&
+              alloc_dev((void**)&w.sec_q[0], 2 * slots * sizeof(SecRay)) &&
+              alloc_dev((void**)&w.sec_q[1], 2 * slots * sizeof(SecRay)) &&
+              alloc_dev((void**)&w.sec_hits, 2 * slots * sizeof(int4));
+    if (ok && caustic) ok = alloc_dev((void**)&w.ev_caustic, ev * sizeof(float4));
+    if (ok && c->n_beam > 0) ok = alloc_dev((void**)&w.beam_add, slots * c->n_beam * sizeof(float4));
+    if (ok && media) ok = alloc_dev((void**)&w.media_p, slots * sizeof(float4)) && alloc_dev((void**)&w.media_r, slots * sizeof(float4));
+    if (!ok) {
+        w.release();
+        sheen_set_error("work buffer allocation failed (%zu slots)", slots);
+        return false;
+    }
+    w.cap_slots = slots; w.cap_L = L; w.cap_beam = c->n_beam; w.cap_caustic = caustic; w.cap_media = media;
+    return true;
+}
+
+struct Timer {
+    SheenContext* c;
+    float* acc;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timer(SheenContext* c_, float* acc_) : c(c_), acc(acc_) {
+        if (!c->profiling) return;
+        (void)hipEventCreate(&a); (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, c->stream);
+    }
+    ~Timer() {
+        if (!c->profiling) return;
+        (void)hipEventRecord(b, c->stream);
+        (void)hipEventSynchronize(b);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        *acc += ms;
+        (void)hipEventDestroy(a); (void)hipEventDestroy(b);
+    }
+};
+
+
+static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st, bool hits_only, int* hk, int* hi,
+                        float* ht) {
+    if (W <= 0 || H <= 0 || aa < 1 || aa > 64) { sheen_set_error("bad render params"); return -1; }
+    if (c->ds.num_materials <= 0) { sheen_set_error("no materials"); return -1; }
+    const int ns = aa;
+    const int tiles_x = (W + 7) / 8, tiles_y = (H + 7) / 8;
+    const size_t total = (size_t)tiles_x * tiles_y * ns * 64;
+    const size_t unit = (size_t)ns * 64;
+    const bool caustic = c->ds.pm.photons && c->ds.caustics > 0.0f;
+    size_t chunk;
+    if (c->chunk_override > 0) chunk = (size_t)c->chunk_override;
+    else {
+        const size_t budget = (size_t)3 << 30;  // 3 GiB of work buffers
+        chunk = budget / slot_bytes(c->ds.num_lights, c->n_beam, caustic, c->ds.num_media > 0);
+        chunk = std::min(chunk, (size_t)8 << 20);
+    }
+    chunk = std::max(unit, chunk / unit * unit);
+    chunk = std::min(chunk, (total + unit - 1) / unit * unit);
+    if (chunk > 0x0FFFFFFF) chunk = 0x0FFFFFFF / unit * unit;
+    if (!ensure_work(c, chunk)) return -1;
+    const size_t npix = (size_t)W * H;
+    if (c->frame_cap < npix) {
+        if (c->frame) (void)hipFree(c->frame);
+        c->frame = nullptr;
+        c->frame_cap = 0;
+        if (!alloc_dev((void**)&c->frame, npix * sizeof(float4))) { sheen_set_error("frame alloc failed"); return -1; }
+        c->frame_cap = npix;
+    }
+    c->frame_w = W; c->frame_h = H;
+    int *dk = nullptr, *di = nullptr;
+    float* dt = nullptr;
+    if (hits_only) {
+        const size_t n = npix * ns;
+        if (!alloc_dev((void**)&dk, n * 4) || !alloc_dev((void**)&di, n * 4) || !alloc_dev((void**)&dt, n * 4)) {
+            sheen_set_error("hit buffer alloc failed");
+            return -1;
+        }
+    }
+    SheenRayStats S{};
+    WorkBufs& w = c->wb;
+    EventBufs EB;
+    EB.base = w.ev_base; EB.caustic = caustic ? w.ev_caustic : nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
+    EB.shadow_q = w.shadow_q; EB.shadow_n = &c->ctr->shadow_n;
+    SampleBufs SB;
+    SB.hit_p = w.hit_p; SB.samp = w.samp; SB.beam_add = c->n_beam ? w.beam_add : nullptr;
+    SB.media_p = c->ds.num_media ? w.media_p : nullptr; SB.media_r = c->ds.num_media ? w.media_r : nullptr;
+    SB.n_beam = c->n_beam;
+    hipStream_t s = c->stream;
+    Counters zero{};
+    HIP_OR(hipMemcpyAsync(c->ctr, &zero, sizeof(Counters), hipMemcpyHostToDevice, s), -1);
+    HIP_OR(hipEventRecord(c->ev_frame[0], s), -1);
+    for (size_t g0 = 0; g0 < total; g0 += chunk) {
+        FrameDesc F;
+        F.width = W; F.height = H; F.ns = ns; F.tiles_x = tiles_x;
+        F.g0 = (uint32_t)g0;
+        F.n_slots = (uint32_t)std::min(chunk, total - g0);
+        k_reset<<<1, 64, 0, s>>>(c->ctr);
+        {
+            Timer t(c, &S.trace_ms);
+            if (c->counting) k_primary<true><<<grid_for(c, F.n_slots, 16), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, c->ctr);
+            else k_primary<false><<<grid_for(c, F.n_slots, 16), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, c->ctr);
+            S.trace_launches++;
+        }
+        if (hits_only) {
+            k_export_hits<<<grid_for(c, F.n_slots, 8), 256, 0, s>>>(F, w.hit_p, dk, di, dt);
+            continue;
+        }
+        {
+            Timer t(c, &S.shade_ms);
+            k_shade_primary<<<grid_for(c, F.n_slots, 8), TRACE_BLOCK, 0, s>>>(c->ds, F, SB, EB, w.sec_q[0], c->ctr);
+        }
+        // three rounds: reflection + first refraction face, then faces 2 and 3
+        for (int round = 0; round < 3; round++) {
+            const int in_q = round & 1, out_q = (round + 1) & 1;
+            const int in_ctr = round, out_ctr = round + 1;
+            const size_t cap = 2 * (size_t)F.n_slots;
+            {
+                Timer t(c, &S.trace_ms);
+                if (c->counting)
+                    k_trace_sec<true><<<grid_for(c, cap, 16), TRACE_BLOCK, 0, s>>>(c->ds, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
+                else
+                    k_trace_sec<false><<<grid_for(c, cap, 16), TRACE_BLOCK, 0, s>>>(c->ds, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
+                S.trace_launches++;
+            }
+            {
+                Timer t(c, &S.shade_ms);
+                k_shade_sec<<<grid_for(c, cap, 8), TRACE_BLOCK, 0, s>>>(c->ds, F, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
+                                                                       w.sec_q[out_q], out_ctr, c->ctr);
+            }
+        }
+        {
+            Timer t(c, &S.shadow_ms);
+            const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
+            if (c->counting) k_shadow<true><<<grid_for(c, cap, 16), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
+            else k_shadow<false><<<grid_for(c, cap, 16), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
+            S.shadow_launches++;
+        }
+        {
+            Timer t(c, &S.shade_ms);
+            k_accumulate<<<grid_for(c, F.n_slots / ns, 8), 256, 0, s>>>(c->ds, F, SB, EB, c->frame);
+        }
+    }
+    HIP_OR(hipEventRecord(c->ev_frame[1], s), -1);
+    HIP_OR(hipGetLastError(), -1);
+    Counters host{};
+    HIP_OR(hipMemcpyAsync(&host, c->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s), -1);
+    HIP_OR(hipStreamSynchronize(s), -1);
+    HIP_OR(hipEventElapsedTime(&S.frame_ms, c->ev_frame[0], c->ev_frame[1]), -1);
+    if (hits_only) {
+        const size_t n = npix * ns;
+        HIP_OR(hipMemcpy(hk, dk, n * 4, hipMemcpyDeviceToHost), -1);
+        HIP_OR(hipMemcpy(hi, di, n * 4, hipMemcpyDeviceToHost), -1);
+        HIP_OR(hipMemcpy(ht, dt, n * 4, hipMemcpyDeviceToHost), -1);
+        (void)hipFree(dk); (void)hipFree(di); (void)hipFree(dt);
+    }
+    S.samples = npix * ns;
+    S.primary_rays = S.samples;
+    S.shadow_rays = host.shadow_passes + host.extra_shadow;
+    S.reflection_rays = host.reflection;
+    S.refraction_rays = host.refraction;
+    S.photon_rays = c->info.photon_rays;
+    S.total_rays = S.primary_rays + S.shadow_rays + S.reflection_rays + S.refraction_rays;
+    S.node_fetches = host.nodes;
+    S.prim_tests = host.prims;
+    S.shadow_node_fetches = host.snodes;
+    S.shadow_prim_tests = host.sprims;
+    c->last = S;
+    if (st) *st = S;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int sheen_init(int device_id, int cuda_arch) {
+    (void)cuda_arch;
+    HIP_OR(hipSetDevice(device_id), -1);
+    HIP_OR(hipFree(nullptr), -1);
+    return 0;
+}
+
+void sheen_cleanup(void) { (void)hipDeviceSynchronize(); }
+
+const char* sheen_get_last_error(void) { return g_error; }
+
+SheenContext* sheen_context_create(const SheenScene* scene) { return context_create(scene); }
+
+void sheen_context_destroy(SheenContext* ctx) { delete ctx; }
+
+int sheen_context_render(SheenContext* ctx, int width, int height, int aa_samples, float* out_rgba,
+                         SheenRayStats* stats) {
+    if (!ctx) { sheen_set_error("null context"); return -1; }
+    if (render_frame(ctx, width, height, aa_samples, stats, false, nullptr, nullptr, nullptr) != 0) return -1;
+    if (out_rgba) HIP_OR(hipMemcpy(out_rgba, ctx->frame, (size_t)width * height * sizeof(float4), hipMemcpyDeviceToHost), -1);
+    return 0;
+}
+
+const float* sheen_context_frame_device(const SheenContext* ctx) { return ctx ? (const float*)ctx->frame : nullptr; }
+
+int sheen_context_primary_hits(SheenContext* ctx, int width, int height, int aa_samples, int* kind, int* index,
+                               float* t) {
+    if (!ctx || !kind || !index || !t) { sheen_set_error("null argument"); return -1; }
+    return render_frame(ctx, width, height, aa_samples, nullptr, true, kind, index, t);
+}
+
+int sheen_context_set_profiling(SheenContext* ctx, int enable) {
+    if (!ctx) return -1;
+    ctx->profiling = enable != 0;
+    return 0;
+}
+int sheen_context_set_counting(SheenContext* ctx, int enable) {
+    if (!ctx) return -1;
+    ctx->counting = enable != 0;
+    return 0;
+}
+int sheen_context_set_chunk(SheenContext* ctx, int samples_per_chunk) {
+    if (!ctx) return -1;
+    ctx->chunk_override = samples_per_chunk > 0 ? samples_per_chunk : 0;
+    return 0;
+}
+
+int sheen_context_photon_stats(SheenContext* ctx, int* emitted, int* stored) {
+    if (!ctx) return -1;
+    if (emitted) *emitted = ctx->info.photons_emitted;
+    if (stored) *stored = ctx->info.photons_stored;
+    return 0;
+}
+
+int sheen_context_photon_map(SheenContext* ctx, Photon* photons, int* cell_start, int* cell_count) {
+    if (!ctx) return -1;
+    if (!ctx->photons) return 0;
+    const int nc = ctx->ds.pm.num_cells;
+    if (photons) HIP_OR(hipMemcpy(photons, ctx->photons, (size_t)ctx->n_photons * sizeof(Photon), hipMemcpyDeviceToHost), -1);
+    if (cell_start) HIP_OR(hipMemcpy(cell_start, ctx->cell_start, (size_t)nc * 4, hipMemcpyDeviceToHost), -1);
+    if (cell_count) HIP_OR(hipMemcpy(cell_count, ctx->cell_count, (size_t)nc * 4, hipMemcpyDeviceToHost), -1);
+    return ctx->n_photons;
+}
+
+int sheen_context_photon_probe(SheenContext* ctx, int stride, float radius, int max_samples, int brute, int* counts) {
+    if (!ctx || stride <= 0 || !counts) { sheen_set_error("bad probe arguments"); return -1; }
+    if (!ctx->photons || ctx->n_photons <= 0) return 0;
+    const uint32_t n = (uint32_t)ctx->n_photons;
+    const uint32_t probes = (n + stride - 1) / stride;
+    int* d = nullptr;
+    if (!alloc_dev((void**)&d, (size_t)probes * 4)) { sheen_set_error("probe alloc failed"); return -1; }
+    k_photon_probe<<<grid_for(ctx, probes, 8), 256, 0, ctx->stream>>>(ctx->ds.pm, ctx->photons, n, stride, radius,
+                                                                        max_samples, d, brute);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(counts, d, (size_t)probes * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) { sheen_set_error("probe failed: %s", hipGetErrorString(e)); return -1; }
+    return (int)probes;
+}
+
+int sheen_render_float(const SheenScene* scene, int width, int height, int aa_samples, float* out_rgba,
+                       SheenRayStats* stats) {
+    if (width <= 0 || height <= 0 || aa_samples < 1 || aa_samples > 64 || !out_rgba) {
+        sheen_set_error("bad render params");
+        return -1;
+    }
+    SheenContext* c = context_create(scene);
+    if (!c) return -1;
+    int rc = sheen_context_render(c, width, height, aa_samples, out_rgba, stats);
+    delete c;
+    return rc;
+}
+
+int sheen_render_scene(const SheenScene* scene, int width, int height, int aa_samples, const char* output_path) {
+    if (!scene) { sheen_set_error("null scene"); return -1; }
+    if (width <= 0 || height <= 0 || aa_samples < 1 || aa_samples > 64 || !output_path) {
+        sheen_set_error("bad render params");
+        return -1;
+    }
+    if (!scene->num_materials) { sheen_set_error("no materials"); return -1; }
+    std::vector<float> rgba((size_t)width * height * 4);
+    if (sheen_render_float(scene, width, height, aa_samples, rgba.data(), nullptr) != 0) return -1;
+    std::vector<unsigned char> rgb((size_t)width * height * 3);
+    sheen_encode_srgb(rgba.data(), width, height, rgb.data());
+    if (sheen_write_png(output_path, width, height, rgb.data()) != 0) {
+        sheen_set_error("png write failed: %s", output_path);
+        return -1;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,55 @@
+// wbvh.h — 8-wide compressed BVH node shared by the host builder and the HIP
+// traversal kernels.
+//
+// One node is exactly one 128-byte HBM/L2 line:
+//   [0,12)   quantisation origin (float x,y,z) = lower corner of the node box
+//   [12,15)  per-axis scale exponent bytes: scale = 2^(e-127), i.e. the float
+//            whose bit pattern is e<<23
+//   [15]     number of used child slots
+//   [16,20)  child_base: node index of the first internal child; internal
+//            children of a node are stored contiguously
+//   [20,24)  prim_base: first entry in the primitive reference array of the
+//            leaf children of this node (stored contiguously, slot order)
+//   [24,32)  meta[8]: 0xFF empty slot; (cnt<<5)|off with cnt in 1..7 for a
+//            leaf of cnt primitives at prim_base+off; (0<<5)|k for an
+//            internal child at child_base+k
+//   [32,128) 16-bit quantised child boxes, SoA: lo_x[8] hi_x[8] lo_y[8]
+//            hi_y[8] lo_z[8] hi_z[8]
+// A child bound decodes as fmaf(float(q), scale, origin).  The builder checks
+// every decoded bound against the exact float bound (same fmaf on the host),
+// so a decoded child box always contains the boxes below it.
+#pragma once
+#include <stdint.h>
+
+#define WBVH_WIDTH 8
+#define WBVH_EMPTY 0xFFu
+#define WBVH_MAX_LEAF 4
+
+struct __attribute__((aligned(128))) WNode {
+    float ox, oy, oz;
+    uint8_t ex, ey, ez, nchild;
+    uint32_t child_base;
+    uint32_t prim_base;
+    uint8_t meta[8];
+    uint16_t q[6][8];
+};
+static_assert(sizeof(WNode) == 128, "WNode must be one 128-byte line");
+
+#ifdef __cplusplus
+#include <vector>
+struct WBvhHost {
+    std::vector<WNode> nodes;
+    std::vector<uint32_t> prims;  // original primitive index per leaf slot
+    int max_stack = 0;            // worst-case traversal stack entries
+    int depth = 0;                // wide levels
+    int leaf_count = 0;
+};
+
+// Builds a wide BVH over n primitive boxes.  bmin/bmax are float[3*n].
+// Binned SAH with `bins` bins over the centroid extent of each axis, leaves of
+// at most max_leaf primitives, then collapsed into 8-wide nodes by repeatedly
+// opening the largest-area internal child.  Primitives listed in `skip`
+// (nullable, n flags) are left out.
+void wbvh_build(const float* bmin, const float* bmax, int n, const unsigned char* skip, int max_leaf, int bins,
+                WBvhHost& out);
+#endif

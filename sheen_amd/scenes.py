@@ -1,0 +1,56 @@
+"""Seeded synthetic workloads (include/sheen_scenes.h) as Python objects."""
+import ctypes as C
+
+from . import native
+
+CONFIGS = {1: "spheres-64-256x256", 2: "mesh-1M-1080p", 5: "shading-10k-512x512",
+           100: "parity-lights", 101: "parity-mesh", 102: "parity-textures"}
+
+
+class GeneratedScene:
+    """Owns a generated scene; ``.scene`` is a ctypes SheenScene pointer."""
+
+    def __init__(self, config, **overrides):
+        lib = native.host()
+        p = native.SheenSceneParams()
+        if lib.sheen_scene_defaults(config, C.byref(p)) != 0:
+            raise ValueError("unknown workload %r" % config)
+        for k, v in overrides.items():
+            setattr(p, k, v)
+        self.params = p
+        self._h = lib.sheen_scene_generate(C.byref(p))
+        if not self._h:
+            raise RuntimeError("scene generation failed for %r" % config)
+        self.scene = lib.sheen_scene_get(self._h)
+        self.width, self.height, self.aa = p.width, p.height, p.aa
+
+    def materials(self):
+        n = C.c_int()
+        ptr = native.host().sheen_scene_materials(self._h, C.byref(n))
+        return [ptr[i] for i in range(n.value)]
+
+    def lights(self):
+        n = C.c_int()
+        ptr = native.host().sheen_scene_lights(self._h, C.byref(n))
+        return [ptr[i] for i in range(n.value)]
+
+    def close(self):
+        if self._h:
+            native.host().sheen_scene_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def generate(config, **overrides):
+    return GeneratedScene(config, **overrides)
