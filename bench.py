@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X render path on the 1M-triangle 1080p workload.
+
+A step is one frame: 1920x1080 pixels x 4 jittered samples of the
+1,048,820-triangle displaced geodesic sphere (70% diffuse / 15% mirror / 15%
+glass, 2 point lights + 1 area light), traced through the full reference path
+(primary, shadow passes, reflection, up to three refraction faces).  `value`
+is every traversal call of the timed frames divided by their wall time, with
+the scene already resident in HBM.
+
+Prints one JSON line (see --help).  --gpus above 1 is refused: the path runs
+on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s, 1M-tri 1080p; batched-vs-serial scene speedup; adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 128   # one wide BVH node per fetch
+TRI_LEAF_BYTES = 48  # vertices + index of one triangle leaf slot
+
+
+def cpu_baseline(scene, width, height, aa, budget_s=20.0):
+    """The oracle (scalar C++ restatement, OpenMP over rows) on the same frame,
+    on the host cores of this box: the whole frame when it fits the time
+    budget, else a band of rows centred on the mesh."""
+    from oracle import Oracle
+    threads = min(16, os.cpu_count() or 1)
+    o = Oracle(scene, threads=threads)
+    probe = 8
+    y0 = height // 2 - probe // 2
+    t = time.perf_counter()
+    o.render(width, height, aa, rows=(y0, y0 + probe))
+    dt = time.perf_counter() - t
+    rows = min(height, max(probe, int(probe * budget_s / max(dt, 1e-3))))
+    y0 = (height - rows) // 2
+    t = time.perf_counter()
+    st = o.render(width, height, aa, rows=(y0, y0 + rows))["stats"]
+    dt = time.perf_counter() - t
+    o.close()
+    what = "the whole %dx%d x%d-spp frame" % (width, height, aa) if rows == height else \
+        "rows %d..%d of the %dx%d x%d-spp frame" % (y0, y0 + rows - 1, width, height, aa)
+    return {
+        "value": st["total_rays"] / dt / 1e6,
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%s (%d samples, %d rays, %.1f s)" % (what, st["samples"], st["total_rays"], dt),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--detail", type=int, default=0, help="geodesic frequency override (default 229)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this render path runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        return 2
+
+    from sheen_amd import scenes, sheen
+
+    kw = {"detail": args.detail} if args.detail else {}
+    sc = scenes.generate(2, **kw)
+    W, H, A = sc.width, sc.height, sc.aa
+    ntri = sc.scene.contents.num_triangles
+    ctx = sheen.Context(sc.scene)
+
+    for _ in range(args.warmup):
+        ctx.render(W, H, A, download=False)
+    # untimed counting pass: nodes fetched / primitives tested per kernel
+    ctx.set_counting(True)
+    _, cst = ctx.render(W, H, A, download=False)
+    ctx.set_counting(False)
+
+    ctx.set_profiling(True)
+    tot = {"rays": 0, "frame_ms": 0.0, "primary_ms": 0.0, "trace_ms": 0.0, "shadow_ms": 0.0, "shade_ms": 0.0,
+           "shadow_launches": 0, "trace_launches": 0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, st = ctx.render(W, H, A, download=False)
+        tot["rays"] += st["total_rays"]
+        for k in ("frame_ms", "primary_ms", "trace_ms", "shadow_ms", "shade_ms", "shadow_launches", "trace_launches"):
+            tot[k] += st[k]
+    t1 = time.perf_counter()
+    ctx.set_profiling(False)
+    wall = t1 - t0
+    value = tot["rays"] / wall / 1e6
+
+    # roofline of the dominant kernel (largest summed time over the timed frames);
+    # one shadow and one primary launch per frame chunk
+    chunks = max(1, tot["shadow_launches"] // args.steps)
+    kernels = {
+        "k_shadow": (tot["shadow_ms"], cst["shadow_node_fetches"] * NODE_BYTES + cst["shadow_prim_tests"] * TRI_LEAF_BYTES),
+        "k_primary": (tot["primary_ms"], cst["primary_node_fetches"] * NODE_BYTES + cst["primary_prim_tests"] * TRI_LEAF_BYTES),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    ms, frame_bytes = kernels[dom]
+    bytes_per_launch = frame_bytes / chunks
+    avg_launch_ms = ms / (chunks * args.steps)
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s_mesh1m.json" % dom)
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded geodesic sphere, fBm displacement)",
+        "config": {
+            "workload": "mesh-1M-1080p: %d triangles, %dx%d, %d spp, 2 point + 1 area light, 30%% mirror/glass"
+                        % (ntri, W, H, A),
+            "triangles": ntri, "width": W, "height": H, "spp": A,
+            "rays_per_frame": tot["rays"] // args.steps,
+            "gpu_frame_ms": round(tot["frame_ms"] / args.steps, 3),
+        },
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_launch": int(bytes_per_launch),
+            "avg_launch_ms": round(avg_launch_ms, 4),
+        },
+        "kernel_ms_per_step": {k: round(tot[k] / args.steps, 3) for k in ("primary_ms", "trace_ms", "shadow_ms",
+                                                                        "shade_ms")},
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(sc.scene, W, H, A)
+        line["cpu_baseline"] = cb
+        line["speedup_vs_cpu"] = round(value / cb["value"], 1)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

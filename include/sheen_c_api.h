@@ -129,11 +129,16 @@ typedef struct SheenRayStats {
     uint64_t prim_tests;
     uint64_t shadow_node_fetches;
     uint64_t shadow_prim_tests;
+    uint64_t primary_node_fetches;   /* of the primary trace kernel only */
+    uint64_t primary_prim_tests;
     /* GPU time from the first to the last kernel of the frame (hipEvents). */
     float frame_ms;
-    /* With profiling enabled: summed durations of the closest-hit trace
-     * kernels and of the shadow trace kernel (hipEvents on the render stream). */
+    /* With profiling enabled (hipEvents on the render stream, no added
+     * synchronisation): summed durations of the closest-hit trace kernels
+     * (primary + secondary), of the primary trace kernel alone, of the shadow
+     * kernel, and of the shading/accumulation kernels. */
     float trace_ms;
+    float primary_ms;
     float shadow_ms;
     float shade_ms;
     int trace_launches;

@@ -93,7 +93,7 @@ struct Counters {
     uint32_t shadow_n;
     uint32_t pad[3];
     unsigned long long shadow_passes;
-    unsigned long long nodes, prims, snodes, sprims;
+    unsigned long long nodes, prims, snodes, sprims, pnodes, pprims;
     unsigned long long reflection, refraction, extra_shadow;
 };
 
@@ -311,6 +311,11 @@ SHD TStack make_stack(uint2* shared) {
 }
 
 // ---------------------------------------------------------------------------
+// Hit record of the wavefront buffers: {t bits, index, carried object id, kind}.
+SHD int4 pack_hit(const QHit& q) {
+    return make_int4(__float_as_int(q.t), q.idx, q.carry, q.kind);
+}
+
 template <bool COUNT>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_primary(DScene S, FrameDesc F, int4* hit_p, Counters* C) {
     __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];
@@ -319,29 +324,31 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_primary(DScene S, FrameDesc F, 
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < F.n_slots; i += stride) {
         int x, y, s;
-        int4 out = make_int4(0, -1, 0, 0);
+        int4 out = make_int4(0, -1, 0, -1);
         if (slot_pixel(F, F.g0 + i, x, y, s)) {
             float pa;
             uint32_t seed;
             DRay r = camera_ray(S.cam, x, y, s, F.ns, F.width, F.height, pa, seed);
             QHit q;
             query_closest<COUNT>(S, r, q, st, ts);
-            out = make_int4(__float_as_int(q.t), q.kind < 0 ? -1 : ((q.kind << 28) | q.idx), q.carry, 0);
+            out = pack_hit(q);
         }
         hit_p[i] = out;
     }
     if (COUNT) {
         wave_add64(&C->nodes, ts.nodes);
         wave_add64(&C->prims, ts.prims);
+        wave_add64(&C->pnodes, ts.nodes);
+        wave_add64(&C->pprims, ts.prims);
     }
 }
 
 SHD QHit unpack_hit(int4 h) {
     QHit q;
     q.t = __int_as_float(h.x);
-    q.kind = h.y < 0 ? -1 : (int)((uint32_t)h.y >> 28);
-    q.idx = h.y < 0 ? -1 : (h.y & 0x0FFFFFFF);
+    q.idx = h.y;
     q.carry = h.z;
+    q.kind = h.w;
     return q;
 }
 
@@ -434,7 +441,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_sec(DScene S, const SecRa
         r.tmax = 1.0e10f;
         QHit q;
         query_closest<COUNT>(S, r, q, st, ts);
-        hits[j] = make_int4(__float_as_int(q.t), q.kind < 0 ? -1 : ((q.kind << 28) | q.idx), q.carry, 0);
+        hits[j] = pack_hit(q);
         if (e.meta & (1u << 28)) nrefr++; else nrefl++;
     }
     wave_add64(&C->reflection, nrefl);
@@ -615,8 +622,8 @@ __global__ void k_export_hits(FrameDesc F, const int4* hit_p, int* kind, int* in
         if (!slot_pixel(F, F.g0 + i, x, y, s)) continue;
         const size_t o = ((size_t)y * F.width + x) * F.ns + s;
         const int4 h = hit_p[i];
-        kind[o] = h.y < 0 ? -1 : (int)((uint32_t)h.y >> 28);
-        index[o] = h.y < 0 ? -1 : (h.y & 0x0FFFFFFF);
-        t[o] = h.y < 0 ? 0.0f : __int_as_float(h.x);
+        kind[o] = h.w;
+        index[o] = h.w < 0 ? -1 : h.y;
+        t[o] = h.w < 0 ? 0.0f : __int_as_float(h.x);
     }
 }

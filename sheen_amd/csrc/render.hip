@@ -84,6 +84,17 @@ struct SheenContext {
     long long chunk_override = 0;
     hipEvent_t ev_frame[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    struct Mark { int slot; hipEvent_t a, b; };
+    std::vector<Mark> marks;
+    hipEvent_t event() {
+        if (ev_used == ev_pool.size()) {
+            hipEvent_t e = nullptr;
+            (void)hipEventCreate(&e);
+            ev_pool.push_back(e);
+        }
+        return ev_pool[ev_used++];
+    }
     SheenRayStats last{};
     Photon* photons = nullptr;  // sorted by cell
     int* cell_start = nullptr;
@@ -339,26 +350,23 @@ static bool ensure_work(SheenContext* c, size_t slots) {
     return true;
 }
 
+// Kernel timing with events recorded on the render stream; elapsed times are
+// read once the frame has finished, so timing adds no synchronisation.
 struct Timer {
     SheenContext* c;
-    float* acc;
-    hipEvent_t a = nullptr, b = nullptr;
-    Timer(SheenContext* c_, float* acc_) : c(c_), acc(acc_) {
+    int slot;
+    hipEvent_t b = nullptr;
+    Timer(SheenContext* c_, int slot_) : c(c_), slot(slot_) {
         if (!c->profiling) return;
-        (void)hipEventCreate(&a); (void)hipEventCreate(&b);
+        hipEvent_t a = c->event();
+        b = c->event();
         (void)hipEventRecord(a, c->stream);
+        c->marks.push_back({slot, a, b});
     }
     ~Timer() {
-        if (!c->profiling) return;
-        (void)hipEventRecord(b, c->stream);
-        (void)hipEventSynchronize(b);
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, a, b);
-        *acc += ms;
-        (void)hipEventDestroy(a); (void)hipEventDestroy(b);
+        if (c->profiling) (void)hipEventRecord(b, c->stream);
     }
 };
-
 
 static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st, bool hits_only, int* hk, int* hi,
                         float* ht) {
@@ -410,6 +418,8 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     hipStream_t s = c->stream;
     Counters zero{};
     HIP_OR(hipMemcpyAsync(c->ctr, &zero, sizeof(Counters), hipMemcpyHostToDevice, s), -1);
+    c->ev_used = 0;
+    c->marks.clear();
     HIP_OR(hipEventRecord(c->ev_frame[0], s), -1);
     for (size_t g0 = 0; g0 < total; g0 += chunk) {
         FrameDesc F;
@@ -418,7 +428,7 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
         F.n_slots = (uint32_t)std::min(chunk, total - g0);
         k_reset<<<1, 64, 0, s>>>(c->ctr);
         {
-            Timer t(c, &S.trace_ms);
+            Timer t(c, 0);
             if (c->counting) k_primary<true><<<grid_for(c, F.n_slots, 16), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, c->ctr);
             else k_primary<false><<<grid_for(c, F.n_slots, 16), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, c->ctr);
             S.trace_launches++;
@@ -428,7 +438,7 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
             continue;
         }
         {
-            Timer t(c, &S.shade_ms);
+            Timer t(c, 3);
             k_shade_primary<<<grid_for(c, F.n_slots, 8), TRACE_BLOCK, 0, s>>>(c->ds, F, SB, EB, w.sec_q[0], c->ctr);
         }
         // three rounds: reflection + first refraction face, then faces 2 and 3
@@ -437,7 +447,7 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
             const int in_ctr = round, out_ctr = round + 1;
             const size_t cap = 2 * (size_t)F.n_slots;
             {
-                Timer t(c, &S.trace_ms);
+                Timer t(c, 1);
                 if (c->counting)
                     k_trace_sec<true><<<grid_for(c, cap, 16), TRACE_BLOCK, 0, s>>>(c->ds, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
                 else
@@ -445,20 +455,20 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
                 S.trace_launches++;
             }
             {
-                Timer t(c, &S.shade_ms);
+                Timer t(c, 3);
                 k_shade_sec<<<grid_for(c, cap, 8), TRACE_BLOCK, 0, s>>>(c->ds, F, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
                                                                        w.sec_q[out_q], out_ctr, c->ctr);
             }
         }
         {
-            Timer t(c, &S.shadow_ms);
+            Timer t(c, 2);
             const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
             if (c->counting) k_shadow<true><<<grid_for(c, cap, 16), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
             else k_shadow<false><<<grid_for(c, cap, 16), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
             S.shadow_launches++;
         }
         {
-            Timer t(c, &S.shade_ms);
+            Timer t(c, 3);
             k_accumulate<<<grid_for(c, F.n_slots / ns, 8), 256, 0, s>>>(c->ds, F, SB, EB, c->frame);
         }
     }
@@ -468,6 +478,14 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     HIP_OR(hipMemcpyAsync(&host, c->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s), -1);
     HIP_OR(hipStreamSynchronize(s), -1);
     HIP_OR(hipEventElapsedTime(&S.frame_ms, c->ev_frame[0], c->ev_frame[1]), -1);
+    for (const auto& m : c->marks) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, m.a, m.b) != hipSuccess) continue;
+        if (m.slot == 0) { S.primary_ms += ms; S.trace_ms += ms; }
+        else if (m.slot == 1) S.trace_ms += ms;
+        else if (m.slot == 2) S.shadow_ms += ms;
+        else S.shade_ms += ms;
+    }
     if (hits_only) {
         const size_t n = npix * ns;
         HIP_OR(hipMemcpy(hk, dk, n * 4, hipMemcpyDeviceToHost), -1);
@@ -484,6 +502,8 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     S.total_rays = S.primary_rays + S.shadow_rays + S.reflection_rays + S.refraction_rays;
     S.node_fetches = host.nodes;
     S.prim_tests = host.prims;
+    S.primary_node_fetches = host.pnodes;
+    S.primary_prim_tests = host.pprims;
     S.shadow_node_fetches = host.snodes;
     S.shadow_prim_tests = host.sprims;
     c->last = S;

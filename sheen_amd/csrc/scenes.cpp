@@ -415,6 +415,40 @@ void gen_p102(Store& st, const SheenSceneParams& p, Rng& rng) {
     st.point_light(f3(-2, 4, -4), f3(1, 1, 1), 500.0f, 0.0f);
 }
 
+// Parity scene 103: the reference's scenes/caustics.sheen restated
+// (glass sphere on a floor under a spot light, 20,000 photons, cell 0.25,
+// caustic scale 0.3, legacy camera with focal_length 0).
+void gen_p103(Store& st, const SheenSceneParams& p) {
+    std::memset(&st.cam, 0, sizeof(st.cam));
+    st.cam.position = f3(0.0f, 1.2f, 4.0f);
+    st.cam.forward = f3(0.0f, -0.1f, -1.0f);
+    st.cam.right = f3(1, 0, 0);
+    st.cam.up = f3(0, 1, 0);
+    int floor_m = st.material(f3(0.2f, 0.2f, 0.22f), 0.1f, 16.0f, 0.1f);
+    int glass = st.material(f3(0.9f, 0.9f, 0.9f), 0.5f, 64.0f, 0.05f);
+    st.plane(f3(0, 1, 0), 0.0f, floor_m);
+    st.sphere(f3(0.0f, 1.0f, 0.0f), 0.5f, glass);
+    st.mats[glass].transmission = 1.0f;
+    st.mats[glass].ior = 1.45f;
+    // spot: angle is the full cone (scene_io.cpp:615-624), direction normalised by the scene builder
+    Light l = Store::base_light();
+    l.type = 3;
+    l.position = f3(0.0f, 3.0f, 2.0f);
+    sheen_f3 d = f3(0.0f, -1.0f, -0.8f);
+    float inv = 1.0f / std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+    l.direction = f3(d.x * inv, d.y * inv, d.z * inv);
+    l.color = f3(1, 1, 1);
+    l.intensity = 200.0f;
+    l.angle = 0.5f * 0.5f;
+    l.blend = 0.2f;
+    l.radius = 0.05f;
+    l.beam = 0.0f;
+    st.lights.push_back(l);
+    st.photons = p.detail > 0 ? p.detail : 20000;
+    st.photon_cell = 0.25f;
+    st.caustics = 0.3f;
+}
+
 }  // namespace
 
 extern "C" {
@@ -429,6 +463,7 @@ int sheen_scene_defaults(int config, SheenSceneParams* out) {
         case 2: out->width = 1920; out->height = 1080; out->aa = 4; out->detail = 229; break;
         case 5: out->width = 512; out->height = 512; out->aa = 8; out->detail = 10000; out->scale = 1.0f; break;
         case 100: case 101: case 102: out->width = 160; out->height = 120; out->aa = 4; out->detail = 0; break;
+        case 103: out->width = 320; out->height = 240; out->aa = 1; out->detail = 20000; break;
         default: return -1;
     }
     return 0;
@@ -447,6 +482,7 @@ void* sheen_scene_generate(const SheenSceneParams* p) {
         case 100: gen_p100(*st, q, rng); break;
         case 101: gen_p101(*st, q, rng); break;
         case 102: gen_p102(*st, q, rng); break;
+        case 103: gen_p103(*st, q); break;
         default: return nullptr;
     }
     st->finalize();

@@ -18,8 +18,9 @@ HOST_PATH = os.path.join(HERE, "libsheen_host.so")
 class SheenRayStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("primary_rays", "shadow_rays", "reflection_rays", "refraction_rays",
                                           "photon_rays", "total_rays", "samples", "node_fetches", "prim_tests",
-                                          "shadow_node_fetches", "shadow_prim_tests")] + \
-               [("frame_ms", C.c_float), ("trace_ms", C.c_float), ("shadow_ms", C.c_float), ("shade_ms", C.c_float),
+                                          "shadow_node_fetches", "shadow_prim_tests", "primary_node_fetches",
+                                          "primary_prim_tests")] + \
+               [("frame_ms", C.c_float), ("trace_ms", C.c_float), ("primary_ms", C.c_float), ("shadow_ms", C.c_float), ("shade_ms", C.c_float),
                 ("trace_launches", C.c_int), ("shadow_launches", C.c_int)]
 
     def as_dict(self):

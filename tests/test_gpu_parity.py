@@ -1,0 +1,132 @@
+"""GPU parity of the HIP render path against the CPU oracle.
+
+Primary hits (category, index, distance) must match bit for bit.  Radiance
+must agree within 1e-4 relative per pixel; pixels whose light samples or
+shadow decisions depend on transcendental functions (sin/cos/tan/pow/exp:
+OCML on the GPU, glibc in the oracle) may flip at a shading discontinuity,
+so those scenes bound the fraction of such pixels instead of requiring none.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# (config, overrides, width, height, aa, max fraction of pixels beyond 1e-4)
+CASES = [
+    ("cfg1", 1, {}, 256, 256, 1, 0.0),
+    ("mesh", 2, {"detail": 40}, 192, 108, 4, 0.0),
+    ("lights", 100, {}, 160, 120, 4, 0.01),
+    ("meshes", 101, {}, 160, 120, 4, 0.0),
+    ("textures", 102, {}, 160, 120, 4, 0.001),
+    ("caustics", 103, {}, 160, 120, 2, 0.01),
+    ("shading", 5, {"detail": 2000}, 128, 128, 2, 0.0),
+]
+
+
+def _rel(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
+
+
+@pytest.fixture(scope="module")
+def renders():
+    from sheen_amd import scenes, sheen
+    from oracle import Oracle
+    out = {}
+    for name, cfg, kw, w, h, aa, tol in CASES:
+        print("case", name, flush=True)
+        sc = scenes.generate(cfg, **kw)
+        with sheen.Context(sc.scene) as ctx:
+            img, st = ctx.render(w, h, aa)
+            kind, idx, t = ctx.primary_hits(w, h, aa)
+        ref = Oracle(sc.scene, threads=min(16, os.cpu_count() or 1)).render(w, h, aa, hits=True)
+        out[name] = dict(img=img, st=st, kind=kind, idx=idx, t=t, ref=ref, tol=tol)
+    return out
+
+
+@pytest.mark.parametrize("name", [c[0] for c in CASES])
+def test_primary_hits_bit_exact(renders, name):
+    r = renders[name]
+    ref = r["ref"]
+    assert np.array_equal(r["kind"], ref["kind"])
+    assert np.array_equal(r["idx"], ref["index"])
+    assert np.array_equal(r["t"].view(np.uint32), ref["t"].view(np.uint32))
+
+
+@pytest.mark.parametrize("name", [c[0] for c in CASES])
+def test_radiance_within_1e4(renders, name):
+    r = renders[name]
+    m = _rel(r["img"][..., :3], r["ref"]["image"][..., :3]).max(-1)
+    frac = float(np.mean(m > 1e-4))
+    print(name, "max rel %.3g, beyond 1e-4: %.5f" % (m.max(), frac))
+    assert frac <= r["tol"]
+
+
+@pytest.mark.parametrize("name", ["cfg1", "mesh", "meshes", "shading"])
+def test_ray_counts_match_oracle(renders, name):
+    st, ref = renders[name]["st"], renders[name]["ref"]["stats"]
+    for k in ("primary_rays", "reflection_rays", "refraction_rays", "shadow_rays"):
+        assert st[k] == ref[k], k
+
+
+def test_chunking_does_not_change_the_image():
+    from sheen_amd import scenes, sheen
+    sc = scenes.generate(101)
+    with sheen.Context(sc.scene) as ctx:
+        a, _ = ctx.render(160, 120, 4)
+        ctx.set_chunk(64 * 4 * 7)
+        b, _ = ctx.render(160, 120, 4)
+    assert np.array_equal(a, b)
+
+
+def test_photon_map_matches_oracle_and_reference_counts():
+    """caustics.sheen: 10,868 of 20,000 photons stored (ROADMAP.md:313);
+    hash-map gather equals brute force (ROADMAP.md:324-325).  Photon positions
+    come through sin/cos of the spot sampling, so they agree with the oracle to
+    rounding, not bit for bit (a photon on a cell boundary may change cell)."""
+    from sheen_amd import scenes, sheen
+    from sheen_amd import contract as K
+    from oracle import Oracle
+    import ctypes as C
+    sc = scenes.generate(103)
+    with sheen.Context(sc.scene) as ctx:
+        emitted, stored = ctx.photon_stats()
+        m = ctx.photon_probe(97, 0.125)
+        b = ctx.photon_probe(97, 0.125, brute=True)
+        gpu = (K.Photon * stored)()
+        assert ctx._lib.sheen_context_photon_map(ctx._c, C.cast(gpu, C.c_void_p), None, None) == stored
+    o = Oracle(sc.scene, threads=4)
+    assert (emitted, stored) == o.photon_stats() == (20000, 10868)
+    assert np.array_equal(m, b)
+    assert 270 <= int(m.mean()) <= 290
+    ref = (K.Photon * stored)()
+    assert o._L.oracle_photon_map(o._h, C.cast(ref, C.c_void_p), None, None) == stored
+    pg = np.ctypeslib.as_array(C.cast(gpu, C.POINTER(C.c_float)), shape=(stored * 12,)).reshape(stored, 12)
+    pr = np.ctypeslib.as_array(C.cast(ref, C.POINTER(C.c_float)), shape=(stored * 12,)).reshape(stored, 12)
+    kg = pg[np.lexsort(pg[:, ::-1].T)]
+    kr = pr[np.lexsort(pr[:, ::-1].T)]
+    np.testing.assert_allclose(kg, kr, rtol=1e-4, atol=1e-5)
+
+
+def test_reference_module_writes_png(tmp_path):
+    """sheen.render_scene (reference api/sheen_py_module.cpp) -> sRGB PNG."""
+    import zlib
+    from sheen_amd import scenes, sheen
+    from oracle import Oracle
+    sc = scenes.generate(100)
+    path = tmp_path / "out.png"
+    sheen.init(0, 0)
+    sheen.render_scene(sc.scene.contents, 64, 48, 2, str(path))
+    data = path.read_bytes()
+    pos, idat = 8, b""
+    while pos < len(data):
+        n = int.from_bytes(data[pos:pos + 4], "big")
+        if data[pos + 4:pos + 8] == b"IDAT":
+            idat += data[pos + 8:pos + 8 + n]
+        pos += 12 + n
+    px = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(48, 1 + 64 * 3)[:, 1:].reshape(48, 64, 3)
+    ref = Oracle(sc.scene, threads=4).render(64, 48, 2)["image"]
+    ref8 = sheen.encode_srgb(ref)
+    diff = np.abs(px.astype(int) - ref8.astype(int))
+    assert diff.max() <= 255 and np.mean(diff > 1) < 0.01
