@@ -415,6 +415,254 @@ void gen_p102(Store& st, const SheenSceneParams& p, Rng& rng) {
     st.point_light(f3(-2, 4, -4), f3(1, 1, 1), 500.0f, 0.0f);
 }
 
+
+// Cubic Bezier runs in groups of four control points, `subdiv` segments per
+// cubic, first point once (host tessellation of the reference's
+// lathe_bezier / prism_bezier, api/scene_io.cpp:66-102).
+std::vector<sheen_f2> bezier_run(const std::vector<sheen_f2>& ctrl, int subdiv) {
+    std::vector<sheen_f2> out;
+    if (ctrl.size() < 4) return out;
+    if (subdiv < 1) subdiv = 1;
+    const int nseg = (int)ctrl.size() / 4;
+    out.push_back(ctrl[0]);
+    for (int s = 0; s < nseg; s++) {
+        const sheen_f2 *P = &ctrl[4 * s];
+        for (int k = 1; k <= subdiv; k++) {
+            float t = (float)k / (float)subdiv, u = 1.0f - t;
+            float w0 = u * u * u, w1 = 3.0f * u * u * t, w2 = 3.0f * u * t * t, w3 = t * t * t;
+            out.push_back({w0 * P[0].x + w1 * P[1].x + w2 * P[2].x + w3 * P[3].x,
+                           w0 * P[0].y + w1 * P[1].y + w2 * P[2].y + w3 * P[3].y});
+        }
+    }
+    return out;
+}
+
+// Image-mode height map of seeded fBm (8-bit, row-major).
+std::vector<unsigned char> fbm_image(int w, int h, uint32_t seed) {
+    std::vector<unsigned char> px((size_t)w * h);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            float v = fbm3(f3(x * 4.0f / w, 0.37f, y * 4.0f / h), 6, seed);
+            px[(size_t)y * w + x] = (unsigned char)std::lround(255.0f * std::fmin(std::fmax(1.6f * (v - 0.2f), 0.0f), 1.0f));
+        }
+    return px;
+}
+
+// Analytic primitives + CSG + caustics (workload 3; `detail` primitives).
+// Half the primitives are CSG-only sphere leaves (Object.type 1, which the
+// sphere BVH skips) in depth-3 trees of 8 leaves under one balanced union
+// root; the rest are spheres (10% glass), superellipsoids (exponent n = 2/e
+// for e in U[0.2,2]), 2-point cone sweeps, multi-point sweeps, 12-point
+// prisms, a few Bezier lathes (the reference ignores a lathe's centre, so
+// they stand on the Y axis) and one fBm height map.
+void gen_cfg3(Store& st, const SheenSceneParams& p, Rng& rng, bool image_hf) {
+    const int N = p.detail > 0 ? p.detail : 10000;
+    const float ext = N >= 2000 ? 24.0f : 8.0f;
+    st.cam = look_at(f3(0.0f, 0.45f * ext + 2.0f, -1.35f * ext), f3(0.0f, 1.0f, 0.0f), 60.0f);
+    std::vector<int> mats;
+    for (int k = 0; k < 48; k++) {
+        int m = st.material(f3(rng.uni(0.1f, 0.9f), rng.uni(0.1f, 0.9f), rng.uni(0.1f, 0.9f)), rng.uni(0.0f, 0.7f),
+                            rng.uni(8.0f, 96.0f), 0.08f);
+        if (k % 8 == 3) st.mats[m].reflect = 0.6f;
+        mats.push_back(m);
+    }
+    int glass = st.material(f3(0.95f, 0.95f, 0.95f), 0.6f, 128.0f, 0.02f);
+    st.mats[glass].transmission = 1.0f;
+    st.mats[glass].ior = 1.5f;
+    auto mat = [&]() { return mats[(rng.next() >> 33) % mats.size()]; };
+    auto pos = [&](float lo, float hi) { return f3(rng.uni(-ext, ext), rng.uni(lo, hi), rng.uni(-ext, ext)); };
+    // ground height map
+    HeightField hf;
+    std::memset(&hf, 0, sizeof(hf));
+    hf.center = f3(0.0f, -0.5f, 0.0f);
+    hf.size_x = hf.size_z = 2.6f * ext;
+    hf.height = 1.5f;
+    hf.freq = 6.0f;
+    hf.octaves = 5;
+    hf.seed = (int)(p.seed & 0x7fffffff);
+    hf.material_id = st.material(f3(0.55f, 0.5f, 0.42f), 0.05f, 8.0f, 0.1f);
+    if (image_hf) {
+        const int R = 1024;
+        st.hf_images.push_back(fbm_image(R, R, (uint32_t)p.seed * 3u + 5u));
+        hf.image = st.hf_images.back().data();
+        hf.img_w = hf.img_h = R;
+        hf.seed = -1;
+    }
+    st.hfs.push_back(hf);
+    int left = N - 1;
+    // CSG: trees of 8 CSG-only spheres
+    const int n_csg = (left / 2) / 8;
+    std::vector<int> roots;
+    for (int t = 0; t < n_csg; t++) {
+        sheen_f3 c = pos(1.0f, 4.0f);
+        int base = (int)st.csg.size();
+        // 7 interior nodes (heap order) then 8 leaves
+        for (int k = 0; k < 7; k++) {
+            CSGNode n; std::memset(&n, 0, sizeof(n));
+            float r = rng.uni();
+            n.op = r < 0.5f ? 0 : (r < 0.75f ? 1 : 2);
+            n.leaf_kind = -1;
+            n.left = base + 2 * k + 1;
+            n.right = base + 2 * k + 2;
+            st.csg.push_back(n);
+        }
+        for (int k = 0; k < 8; k++) {
+            Object o; std::memset(&o, 0, sizeof(o));
+            o.type = 1;
+            o.material_id = mat();
+            o.param1 = add(c, f3(rng.uni(-0.6f, 0.6f), rng.uni(-0.6f, 0.6f), rng.uni(-0.6f, 0.6f)));
+            o.param2 = rng.uni(0.3f, 0.8f);
+            st.objects.push_back(o);
+            CSGNode n; std::memset(&n, 0, sizeof(n));
+            n.op = 0; n.leaf_kind = 0; n.leaf_index = (int)st.objects.size() - 1; n.left = n.right = -1;
+            st.csg.push_back(n);
+        }
+        roots.push_back(base);
+        left -= 8;
+    }
+    while (roots.size() > 1) {  // balanced union of the trees
+        std::vector<int> next;
+        for (size_t i = 0; i + 1 < roots.size(); i += 2) {
+            CSGNode n; std::memset(&n, 0, sizeof(n));
+            n.op = 0; n.leaf_kind = -1; n.left = roots[i]; n.right = roots[i + 1];
+            st.csg.push_back(n);
+            next.push_back((int)st.csg.size() - 1);
+        }
+        if (roots.size() % 2) next.push_back(roots.back());
+        roots.swap(next);
+    }
+    st.csg_root = roots.empty() ? -1 : roots[0];
+    // standalone primitives
+    const int n_se = left / 5, n_cone = left / 10, n_sweep = left / 10, n_prism = left * 4 / 25;
+    const int n_lathe = std::max(2, std::min(20, left / 250));
+    for (int i = 0; i < n_se; i++) {
+        Superellipsoid e; std::memset(&e, 0, sizeof(e));
+        e.center = pos(0.8f, 4.0f);
+        e.a = rng.uni(0.2f, 0.6f); e.b = rng.uni(0.2f, 0.6f); e.c = rng.uni(0.2f, 0.6f);
+        e.n = 2.0f / rng.uni(0.2f, 2.0f);
+        e.material_id = mat();
+        st.ses.push_back(e);
+    }
+    auto strand = [&](int npts, bool cone) {
+        SweepStrand s; std::memset(&s, 0, sizeof(s));
+        s.point_offset = (int)st.sweep_points.size();
+        s.point_count = npts;
+        s.material_id = mat();
+        sheen_f3 q = pos(0.5f, 4.0f);
+        for (int k = 0; k < npts; k++) {
+            SweepPoint sp; std::memset(&sp, 0, sizeof(sp));
+            sp.position = q;
+            sp.radius = cone ? (k == 0 ? rng.uni(0.2f, 0.5f) : rng.uni(0.02f, 0.1f)) : rng.uni(0.05f, 0.15f);
+            st.sweep_points.push_back(sp);
+            q = add(q, f3(rng.uni(-0.6f, 0.6f), rng.uni(0.2f, 0.8f), rng.uni(-0.6f, 0.6f)));
+        }
+        st.strands.push_back(s);
+    };
+    for (int i = 0; i < n_cone; i++) strand(2, true);
+    for (int i = 0; i < n_sweep; i++) strand(4 + (int)(rng.next() % 3), false);
+    for (int i = 0; i < n_prism; i++) {
+        Prism pr; std::memset(&pr, 0, sizeof(pr));
+        pr.center = pos(0.0f, 0.0f);
+        pr.y_min = rng.uni(0.2f, 1.0f);
+        pr.y_max = pr.y_min + rng.uni(0.2f, 1.5f);
+        pr.point_offset = (int)st.prism_points.size();
+        pr.point_count = 13;
+        pr.material_id = mat();
+        float r0 = rng.uni(0.2f, 0.6f), r1 = r0 * rng.uni(0.3f, 0.9f), ph = rng.uni(0.0f, 6.2831853f);
+        PrismPoint first{};
+        for (int k = 0; k < 12; k++) {
+            float a = ph + 6.2831853f * k / 12.0f, rr = (k % 2) ? r1 : r0;
+            PrismPoint q; std::memset(&q, 0, sizeof(q));
+            q.x = rr * std::cos(a); q.z = rr * std::sin(a);
+            if (k == 0) first = q;
+            st.prism_points.push_back(q);
+        }
+        st.prism_points.push_back(first);
+        st.prisms.push_back(pr);
+    }
+    for (int i = 0; i < n_lathe; i++) {
+        std::vector<sheen_f2> ctrl;
+        float y = 0.0f, r = rng.uni(0.3f, 0.8f) * (1.0f + i * 0.15f);
+        for (int s = 0; s < 8; s++) {
+            float y3 = y + rng.uni(0.15f, 0.35f), r3 = std::fmax(0.05f, r + rng.uni(-0.25f, 0.25f));
+            ctrl.push_back({y, r});
+            ctrl.push_back({y + (y3 - y) * 0.33f, r + rng.uni(-0.2f, 0.2f)});
+            ctrl.push_back({y + (y3 - y) * 0.66f, r3 + rng.uni(-0.2f, 0.2f)});
+            ctrl.push_back({y3, r3});
+            y = y3; r = r3;
+        }
+        std::vector<sheen_f2> line = bezier_run(ctrl, 4);
+        Lathe l; std::memset(&l, 0, sizeof(l));
+        l.center = f3(0, 0, 0);
+        l.point_offset = (int)st.lathe_points.size();
+        l.point_count = (int)line.size();
+        l.material_id = mat();
+        for (auto& q : line) {
+            LathePoint lp; std::memset(&lp, 0, sizeof(lp));
+            lp.y = q.x; lp.radius = q.y;
+            st.lathe_points.push_back(lp);
+        }
+        st.lathes.push_back(l);
+    }
+    left -= n_se + n_cone + n_sweep + n_prism + n_lathe;
+    for (int i = 0; i < left; i++) {
+        int m = (i % 10 == 0) ? glass : mat();
+        st.sphere(pos(0.6f, 4.5f), rng.uni(0.15f, 0.7f), m);
+    }
+    st.spot_light(f3(0.0f, 0.9f * ext, -0.2f * ext), f3(0.0f, -1.0f, 0.2f), f3(1.0f, 0.95f, 0.85f),
+                  120.0f * ext * ext, 1.2f, 0.3f, 0.1f, 0.0f);
+    st.point_light(f3(-0.6f * ext, 0.5f * ext, -0.6f * ext), f3(0.7f, 0.8f, 1.0f), 40.0f * ext * ext, 0.0f);
+    st.sun_light(f3(0.3f, -1.0f, 0.4f), f3(1.0f, 1.0f, 1.0f), 0.4f, 0.0f);
+    st.photons = p.scale > 0 ? (int)p.scale : (N >= 2000 ? 550000 : 20000);
+    st.photon_cell = 0.25f;
+    st.caustics = 0.3f;
+}
+
+// Parity scene 105: hair strands with optical-depth shadows, normal maps,
+// spot gobo, sun with an angular radius, emission, a scattering medium and a
+// visible spot beam.
+void gen_p105(Store& st, const SheenSceneParams& p, Rng& rng) {
+    st.cam = look_at(f3(0.0f, 1.6f, -4.5f), f3(0.0f, 0.6f, 0.0f), 55.0f);
+    int floor_m = st.material(f3(0.6f, 0.6f, 0.6f), 0.1f, 16.0f, 0.1f);
+    int tex = st.texture(32, 32, noise_texture(32, 32, 91u + (uint32_t)p.seed));
+    st.mats[floor_m].normal_id = tex;
+    st.mats[floor_m].normal_strength = 0.5f;
+    st.plane(f3(0, 1, 0), 0.0f, floor_m);
+    int skin = st.material(f3(0.75f, 0.6f, 0.5f), 0.25f, 24.0f, 0.15f);
+    st.sphere(f3(0.0f, 0.8f, 0.5f), 0.5f, skin);
+    int hair = st.material(f3(0.35f, 0.22f, 0.12f), 0.6f, 128.0f, 0.1f);
+    Material& hm = st.mats[hair];
+    hm.anisotropy = 1.0f; hm.hair_shift = 0.12f; hm.hair_spec_r = 1.2f; hm.hair_exp_r = 12.0f;
+    hm.hair_spec_t = 0.5f; hm.hair_exp_t = 6.0f; hm.hair_backlight = 0.5f;
+    for (int k = 0; k < 60; k++) {
+        SweepStrand s; std::memset(&s, 0, sizeof(s));
+        s.point_offset = (int)st.sweep_points.size();
+        s.point_count = 6;
+        s.material_id = hair;
+        float a = rng.uni(0.0f, 6.2831853f), b = rng.uni(0.2f, 1.3f);
+        sheen_f3 q = add(f3(0.0f, 0.8f, 0.5f), mul(f3(std::cos(a) * std::sin(b), std::cos(b), std::sin(a) * std::sin(b)), 0.52f));
+        for (int j = 0; j < 6; j++) {
+            SweepPoint sp; std::memset(&sp, 0, sizeof(sp));
+            sp.position = q;
+            sp.radius = 0.012f * (1.0f - 0.1f * j);
+            st.sweep_points.push_back(sp);
+            q = add(q, f3(std::cos(a) * 0.08f, -0.09f, std::sin(a) * 0.08f));
+        }
+        st.strands.push_back(s);
+    }
+    int lamp = st.material(f3(1, 1, 1), 0.0f, 8.0f, 0.0f);
+    st.mats[lamp].emission_color = f3(1.0f, 0.8f, 0.5f);
+    st.mats[lamp].emission = 3.0f;
+    st.sphere(f3(1.4f, 0.25f, 0.2f), 0.25f, lamp);
+    st.sun_light(f3(-0.3f, -1.0f, 0.5f), f3(1.0f, 0.95f, 0.9f), 0.5f, 0.04f);
+    st.spot_light(f3(-1.2f, 2.6f, -0.6f), f3(0.5f, -1.0f, 0.4f), f3(1.0f, 1.0f, 0.9f), 300.0f, 0.6f, 0.3f, 0.05f, 0.8f);
+    st.lights.back().texture_id = tex;
+    Medium md; std::memset(&md, 0, sizeof(md));
+    md.center = f3(-0.8f, 0.8f, 0.0f); md.half = f3(0.8f, 0.8f, 0.8f);
+    md.sigma_s = f3(0.3f, 0.3f, 0.35f); md.sigma_t = 0.4f; md.samples = 8;
+    st.media.push_back(md);
+}
+
 // Parity scene 103: the reference's scenes/caustics.sheen restated
 // (glass sphere on a floor under a spot light, 20,000 photons, cell 0.25,
 // caustic scale 0.3, legacy camera with focal_length 0).
@@ -464,6 +712,9 @@ int sheen_scene_defaults(int config, SheenSceneParams* out) {
         case 5: out->width = 512; out->height = 512; out->aa = 8; out->detail = 10000; out->scale = 1.0f; break;
         case 100: case 101: case 102: out->width = 160; out->height = 120; out->aa = 4; out->detail = 0; break;
         case 103: out->width = 320; out->height = 240; out->aa = 1; out->detail = 20000; break;
+        case 3: out->width = 1280; out->height = 720; out->aa = 16; out->detail = 10000; break;
+        case 104: out->width = 160; out->height = 90; out->aa = 4; out->detail = 300; break;
+        case 105: out->width = 160; out->height = 120; out->aa = 4; out->detail = 0; break;
         default: return -1;
     }
     return 0;
@@ -483,6 +734,9 @@ void* sheen_scene_generate(const SheenSceneParams* p) {
         case 101: gen_p101(*st, q, rng); break;
         case 102: gen_p102(*st, q, rng); break;
         case 103: gen_p103(*st, q); break;
+        case 3: gen_cfg3(*st, q, rng, true); break;
+        case 104: gen_cfg3(*st, q, rng, false); break;
+        case 105: gen_p105(*st, q, rng); break;
         default: return nullptr;
     }
     st->finalize();

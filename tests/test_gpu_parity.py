@@ -22,6 +22,8 @@ CASES = [
     ("textures", 102, {}, 160, 120, 4, 0.001),
     ("caustics", 103, {}, 160, 120, 2, 0.01),
     ("shading", 5, {"detail": 2000}, 128, 128, 2, 0.0),
+    ("analytic", 104, {}, 160, 90, 4, 0.05),
+    ("features", 105, {}, 160, 120, 4, 0.05),
 ]
 
 
