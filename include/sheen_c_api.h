@@ -131,6 +131,8 @@ typedef struct SheenRayStats {
     uint64_t shadow_prim_tests;
     uint64_t primary_node_fetches;   /* of the primary trace kernel only */
     uint64_t primary_prim_tests;
+    uint64_t primary_stack_spills;   /* traversal-stack entries moved out of LDS */
+    uint64_t shadow_stack_spills;
     /* GPU time from the first to the last kernel of the frame (hipEvents). */
     float frame_ms;
     /* With profiling enabled (hipEvents on the render stream, no added
@@ -143,6 +145,7 @@ typedef struct SheenRayStats {
     float shade_ms;
     int trace_launches;
     int shadow_launches;
+    float shade_primary_ms;          /* the primary-hit shading kernel alone */
 } SheenRayStats;
 
 typedef struct SheenContext SheenContext;
@@ -175,6 +178,12 @@ int sheen_context_set_profiling(SheenContext* ctx, int enable);
 int sheen_context_set_counting(SheenContext* ctx, int enable);
 /* Samples per wavefront chunk (0 = automatic). */
 int sheen_context_set_chunk(SheenContext* ctx, int samples_per_chunk);
+/* Launch-shape parameters set by the autotuner: "primary_grid", "sec_grid",
+ * "shadow_grid", "shade_grid", "shade_sec_grid" (blocks per CU of each
+ * grid-stride kernel; 0 = one work item per thread) and "chunk" (samples per
+ * chunk, 0 = automatic).  Returns 0, or -1 for an unknown name or a negative
+ * value. */
+int sheen_context_set_param(SheenContext* ctx, const char* name, int value);
 
 /* Photon pass results of a context (scene.photons > 0): photons emitted and
  * stored (reference render_kernel.cu:928-953). */

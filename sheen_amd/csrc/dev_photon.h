@@ -13,8 +13,7 @@ struct PhotonCounters {
 
 __global__ __launch_bounds__(TRACE_BLOCK) void k_photon_emit(DScene S, int per_light, uint32_t total, Photon* out,
                                                              uint32_t* flag, PhotonCounters* pc) {
-    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];
-    TStack st = make_stack(s_stack);
+    SHEEN_STACK(st);
     TraceStats ts{0, 0};
     uint32_t rays = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -57,7 +56,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_photon_emit(DScene S, int per_l
         for (int bounce = 0; bounce < 4; bounce++) {
             QHit q;
             rays++;
-            query_closest<false>(S, ray, q, st, ts);
+            query_closest<false, true>(S, ray, q, st, ts);
             if (q.kind < 0) break;
             DHit h;
             rebuild_hit(S, ray, q, h);

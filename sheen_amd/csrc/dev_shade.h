@@ -169,6 +169,36 @@ SHD uint32_t wave_reserve(uint32_t* counter, bool want) {
     return want ? base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)) : 0xFFFFFFFFu;
 }
 
+// Block-wide reservation of up to two queue entries per thread (flags a, b)
+// with one atomic per block: device-scope atomics are performed past the
+// per-XCD L2, so a per-wave atomic for every light and ray kind serialises the
+// shading kernels.  Every thread of the block must make the call at the same
+// point (block-uniform control flow).  A thread's entries are base (if a) and
+// base + a (if b); returns 0xFFFFFFFF when it asks for none.
+SHD uint32_t block_reserve(uint32_t* counter, bool a, bool b) {
+    __shared__ uint32_t s_res[TRACE_BLOCK / 64 + 1];
+    const uint64_t ma = __ballot(a), mb = __ballot(b);
+    const int lane = (int)__lane_id();
+    const int wave = (int)(threadIdx.x >> 6);
+    const int nw = (int)(blockDim.x >> 6);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    if (lane == 0) s_res[wave] = (uint32_t)(__popcll(ma) + __popcll(mb));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < nw; w++) {
+            const uint32_t c = s_res[w];
+            s_res[w] = tot;
+            tot += c;
+        }
+        s_res[nw] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = s_res[nw] + s_res[wave] + (uint32_t)(__popcll(ma & lt) + __popcll(mb & lt));
+    __syncthreads();  // s_res is reused by the next call
+    return (a || b) ? base : 0xFFFFFFFFu;
+}
+
 // Shadow-ray request: 32 bytes.
 struct ShadowReq {
     float ox, oy, oz, tmax;
@@ -350,7 +380,7 @@ SHD void shade_event(const DScene& S, bool active, const DHit& rec, v3 view_dir,
             B.contrib[(size_t)ev * L + i] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
             B.trans[(size_t)ev * L + i] = 0.0f;
         }
-        const uint32_t q = wave_reserve(B.shadow_n, emit);
+        const uint32_t q = block_reserve(B.shadow_n, emit, false);
         if (emit) {
             ShadowReq r;
             r.ox = so.x; r.oy = so.y; r.oz = so.z; r.tmax = stmax;

@@ -17,9 +17,29 @@
 #pragma once
 #include "dev_prims.h"
 
-#define LDS_DEPTH 16
-#define SPILL_DEPTH 112
+// Stack entries kept in LDS per lane, and the waves-per-SIMD target of the
+// traversal kernels (0: compiler's choice); build-time tuning knobs.
+#ifndef LDS_DEPTH
+#define LDS_DEPTH 8
+#endif
+#ifndef TRACE_MIN_WAVES
+#define TRACE_MIN_WAVES 6
+#endif
+#define SPILL_DEPTH (128 - LDS_DEPTH)
 #define TRACE_BLOCK 256
+#if TRACE_MIN_WAVES > 0
+#define TRACE_BOUNDS __launch_bounds__(TRACE_BLOCK, TRACE_MIN_WAVES)
+#else
+#define TRACE_BOUNDS __launch_bounds__(TRACE_BLOCK)
+#endif
+#ifndef SHADE_MIN_WAVES
+#define SHADE_MIN_WAVES 0
+#endif
+#if SHADE_MIN_WAVES > 0
+#define SHADE_BOUNDS __launch_bounds__(TRACE_BLOCK, SHADE_MIN_WAVES)
+#else
+#define SHADE_BOUNDS __launch_bounds__(TRACE_BLOCK)
+#endif
 #define CULL_MARGIN 0.0078125f
 
 // 19-comparator sorting network for 8 keys (ascending).
@@ -37,25 +57,30 @@ SHD void sort8(uint32_t* k) {
 }
 
 // Per-lane traversal stack: the top LDS_DEPTH entries live in LDS (one
-// column per lane, conflict-free), older entries spill to private memory.
+// column per lane, conflict-free), older entries spill to a private array
+// owned by the kernel.  The struct holds only pointers and counters so it
+// stays in registers; the stack pointer itself is a local of traverse().
 struct TStack {
-    uint2* lds;  // &shared[tid], stride TRACE_BLOCK
-    uint2 spill[SPILL_DEPTH];
-    int sp;
-    SHD void push(uint32_t node, float t) {
-        int slot = sp % LDS_DEPTH;
-        if (sp >= LDS_DEPTH) spill[sp - LDS_DEPTH] = lds[slot * TRACE_BLOCK];
-        lds[slot * TRACE_BLOCK] = make_uint2(node, __float_as_uint(t));
-        sp++;
-    }
-    SHD uint2 pop() {
-        sp--;
-        int slot = sp % LDS_DEPTH;
-        uint2 e = lds[slot * TRACE_BLOCK];
-        if (sp >= LDS_DEPTH) lds[slot * TRACE_BLOCK] = spill[sp - LDS_DEPTH];
-        return e;
-    }
+    uint2* lds;    // &shared[tid], stride TRACE_BLOCK
+    uint2* spill;  // private array of SPILL_DEPTH entries
+    uint32_t spills;
 };
+SHD void stack_push(TStack& st, int& sp, uint32_t node, float t) {
+    const int slot = sp % LDS_DEPTH;
+    if (sp >= LDS_DEPTH) {
+        st.spill[sp - LDS_DEPTH] = st.lds[slot * TRACE_BLOCK];
+        st.spills++;
+    }
+    st.lds[slot * TRACE_BLOCK] = make_uint2(node, __float_as_uint(t));
+    sp++;
+}
+SHD uint2 stack_pop(TStack& st, int& sp) {
+    sp--;
+    const int slot = sp % LDS_DEPTH;
+    uint2 e = st.lds[slot * TRACE_BLOCK];
+    if (sp >= LDS_DEPTH) st.lds[slot * TRACE_BLOCK] = st.spill[sp - LDS_DEPTH];
+    return e;
+}
 
 struct TraceStats {
     uint32_t nodes, prims;
@@ -110,7 +135,7 @@ template <int CAT, bool ANY, bool COUNT>
 SHD void traverse(const DScene& S, const DRay& r, v3 inv, float& cat_t, int& cat_idx, bool& cat_hit, TStack& st,
                   TraceStats& ts) {
     const WNode* nodes = S.bvh[CAT];
-    st.sp = 0;
+    int sp = 0;
     uint32_t node = 0;
     for (;;) {
         const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
@@ -172,7 +197,7 @@ SHD void traverse(const DScene& S, const DRay& r, v3 inv, float& cat_t, int& cat
             for (int j = 7; j >= 1; j--) {
                 if (keys[j] != 0xFFFFFFFFu) {
                     const uint32_t m = (uint32_t)(meta >> (8 * (keys[j] & 7u))) & 0xFFu;
-                    if (st.sp < LDS_DEPTH + SPILL_DEPTH) st.push(child_base + (m & 31u), __uint_as_float(keys[j] & ~7u));
+                    if (sp < LDS_DEPTH + SPILL_DEPTH) stack_push(st, sp, child_base + (m & 31u), __uint_as_float(keys[j] & ~7u));
                 }
             }
             const uint32_t m0 = (uint32_t)(meta >> (8 * (keys[0] & 7u))) & 0xFFu;
@@ -180,8 +205,8 @@ SHD void traverse(const DScene& S, const DRay& r, v3 inv, float& cat_t, int& cat
             continue;
         }
         bool found = false;
-        while (st.sp > 0) {
-            uint2 e = st.pop();
+        while (sp > 0) {
+            uint2 e = stack_pop(st, sp);
             if (__uint_as_float(e.y) <= cat_t + cat_t * CULL_MARGIN) {
                 node = e.x;
                 found = true;
@@ -287,8 +312,10 @@ SHD void closest_cat(const DScene& S, const DRay& r, v3 inv, float init_t, QHit&
     }
 }
 
-// scene_intersect (reference integrator/raycast.h:14-70).
-template <bool COUNT>
+// scene_intersect (reference integrator/raycast.h:14-70).  FULL = false
+// compiles only the sphere, triangle and plane categories (scenes without
+// other kinds use that instantiation).
+template <bool COUNT, bool FULL = true>
 SHD void query_closest(const DScene& S, const DRay& r, QHit& q, TStack& st, TraceStats& ts) {
     const v3 inv = mk3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
     q.t = 0.0f; q.kind = -1; q.idx = -1; q.carry = 0;
@@ -300,14 +327,16 @@ SHD void query_closest(const DScene& S, const DRay& r, QHit& q, TStack& st, Trac
     const bool exact_carry = (S.csg_root >= 0) && (S.num_triangles > 0);
 #define SHEEN_INIT (exact_carry ? r.tmax : best)
     closest_cat<C_SPHERE, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
-    closest_cat<C_STRAND, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+    if (FULL) closest_cat<C_STRAND, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
     closest_cat<C_TRI, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
-    closest_cat<C_HF, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
-    closest_cat<C_SE, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
-    closest_cat<C_LATHE, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
-    closest_cat<C_PRISM, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+    if (FULL) {
+        closest_cat<C_HF, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+        closest_cat<C_SE, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+        closest_cat<C_LATHE, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+        closest_cat<C_PRISM, COUNT>(S, r, inv, SHEEN_INIT, q, best, last_obj, st, ts);
+    }
 #undef SHEEN_INIT
-    if (S.csg_root >= 0) {
+    if (FULL && S.csg_root >= 0) {
         CSpan iv;
         float t;
         if (csg_query(S, r, r.tmin, r.tmax, iv, t) && t < best) {
@@ -336,17 +365,19 @@ SHD bool opaque_cat(const DScene& S, const DRay& r, v3 inv, QHit& q, TStack& st,
 // scene_intersect_opaque (reference integrator/raycast.h:74-97).  For a
 // category whose materials can never pass a shadow ray the first hit found
 // decides (the closest hit of that category would block as well).
-template <bool COUNT>
+template <bool COUNT, bool FULL = true>
 SHD bool query_opaque(const DScene& S, const DRay& r, QHit& q, TStack& st, TraceStats& ts) {
     const v3 inv = mk3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
     q.kind = -1; q.idx = -1; q.carry = 0; q.t = 0.0f;
     if (opaque_cat<C_SPHERE, COUNT>(S, r, inv, q, st, ts)) return true;
     if (opaque_cat<C_TRI, COUNT>(S, r, inv, q, st, ts)) return true;
-    if (opaque_cat<C_HF, COUNT>(S, r, inv, q, st, ts)) return true;
-    if (opaque_cat<C_SE, COUNT>(S, r, inv, q, st, ts)) return true;
-    if (opaque_cat<C_LATHE, COUNT>(S, r, inv, q, st, ts)) return true;
-    if (opaque_cat<C_PRISM, COUNT>(S, r, inv, q, st, ts)) return true;
-    if (S.csg_root >= 0) {
+    if (FULL) {
+        if (opaque_cat<C_HF, COUNT>(S, r, inv, q, st, ts)) return true;
+        if (opaque_cat<C_SE, COUNT>(S, r, inv, q, st, ts)) return true;
+        if (opaque_cat<C_LATHE, COUNT>(S, r, inv, q, st, ts)) return true;
+        if (opaque_cat<C_PRISM, COUNT>(S, r, inv, q, st, ts)) return true;
+    }
+    if (FULL && S.csg_root >= 0) {
         CSpan iv;
         float t;
         if (csg_query(S, r, r.tmin, r.tmax, iv, t)) { q.t = t; q.kind = C_CSG; q.idx = 0; q.carry = 0; return true; }
@@ -362,6 +393,7 @@ SHD bool query_opaque(const DScene& S, const DRay& r, QHit& q, TStack& st, Trace
 }
 
 // Material of a hit without rebuilding the record.
+template <bool FULL = true>
 SHD int hit_material(const DScene& S, const QHit& q, const DRay& r) {
     switch (q.kind) {
         case C_SPHERE: return S.objects[q.idx].material_id;
@@ -372,6 +404,7 @@ SHD int hit_material(const DScene& S, const QHit& q, const DRay& r) {
         case C_LATHE: return S.lathes[q.idx].material_id;
         case C_PRISM: return S.prisms[q.idx].material_id;
         case C_CSG: {
+            if (!FULL) return 0;
             CSpan iv;
             float t;
             csg_query(S, r, r.tmin, r.tmax, iv, t);
@@ -384,6 +417,7 @@ SHD int hit_material(const DScene& S, const QHit& q, const DRay& r) {
 
 // Full hit record of the winning primitive: its test is run again on the
 // same ray, which reproduces the distance and every attribute exactly.
+template <bool FULL = true>
 SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
     h.kind = q.kind;
     h.index = q.idx;
@@ -397,7 +431,7 @@ SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
             break;
         }
         case C_STRAND:
-            hit_strand(r, S.strands[q.idx], S.sweep_points, h);
+            if (FULL) hit_strand(r, S.strands[q.idx], S.sweep_points, h);
             h.object_id = -(q.idx + 1);
             break;
         case C_TRI:
@@ -405,22 +439,23 @@ SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
             h.material_id = S.triangles[q.idx].material_id; h.object_id = -(q.idx + 1);
             break;
         case C_HF:
-            hit_hf(r, S.hfs[q.idx], h);
+            if (FULL) hit_hf(r, S.hfs[q.idx], h);
             h.material_id = S.hfs[q.idx].material_id; h.object_id = -(q.idx + 1);
             break;
         case C_SE:
-            hit_se(r, S.ses[q.idx], h);
+            if (FULL) hit_se(r, S.ses[q.idx], h);
             h.material_id = S.ses[q.idx].material_id; h.object_id = -(q.idx + 1);
             break;
         case C_LATHE:
-            hit_lathe(r, S.lathes[q.idx], S.lathe_points, h);
+            if (FULL) hit_lathe(r, S.lathes[q.idx], S.lathe_points, h);
             h.object_id = -(q.idx + 1);
             break;
         case C_PRISM:
-            hit_prism(r, S.prisms[q.idx], S.prism_points, h);
+            if (FULL) hit_prism(r, S.prisms[q.idx], S.prism_points, h);
             h.object_id = -(q.idx + 1);
             break;
         case C_CSG: {
+            if (!FULL) break;
             CSpan iv;
             float t;
             csg_query(S, r, r.tmin, r.tmax, iv, t);

@@ -195,7 +195,7 @@ bool host_prepare_scene(const SheenScene& s, HostScene& out, std::string& err) {
             err = e.what();
             return false;
         }
-        if (wb.max_stack > 16 + 112) { err = "BVH too deep for the traversal stack"; return false; }
+        if (wb.max_stack > 128) { err = "BVH too deep for the traversal stack"; return false; }
         hb.nodes.swap(wb.nodes);
         hb.prims.swap(wb.prims);
         hb.max_stack = wb.max_stack;

@@ -95,6 +95,7 @@ struct Counters {
     unsigned long long shadow_passes;
     unsigned long long nodes, prims, snodes, sprims, pnodes, pprims;
     unsigned long long reflection, refraction, extra_shadow;
+    unsigned long long spills, sspills;
 };
 
 SHD uint32_t wave_sum(uint32_t v) {
@@ -107,7 +108,7 @@ SHD void wave_add64(unsigned long long* dst, uint32_t v) {
 }
 
 // Shadow test of one request (reference render_kernel.cu:237-280).
-template <bool COUNT>
+template <bool COUNT, bool FULL>
 SHD float shadow_eval(const DScene& S, const ShadowReq& q, TStack& st, uint32_t& passes, TraceStats& ts) {
     DRay r;
     r.o = mk3(q.ox, q.oy, q.oz);
@@ -119,12 +120,12 @@ SHD float shadow_eval(const DScene& S, const ShadowReq& q, TStack& st, uint32_t&
     for (int pass = 0; pass < 3; pass++) {
         passes++;
         QHit h;
-        if (!query_opaque<COUNT>(S, r, h, st, ts)) { blocked = false; break; }
-        const Material& m = S.materials[hit_material(S, h, r)];
+        if (!query_opaque<COUNT, FULL>(S, r, h, st, ts)) { blocked = false; break; }
+        const Material& m = S.materials[hit_material<FULL>(S, h, r)];
         bool through = false;
         if (m.transmission > 0.5f) {
             DHit full;
-            rebuild_hit(S, r, h, full);
+            rebuild_hit<FULL>(S, r, h, full);
             if (transmission_at(S, full.uvx, full.uvy, m) > 0.5f) {
                 r.o = full.point + r.d * 0.002f;
                 through = true;
@@ -133,7 +134,7 @@ SHD float shadow_eval(const DScene& S, const ShadowReq& q, TStack& st, uint32_t&
         if (!through) { blocked = true; break; }
     }
     if (blocked) return 0.0f;
-    if (q.slot & 0x80000000u) return expf(-strands_depth(S, orig));
+    if (FULL && (q.slot & 0x80000000u)) return expf(-strands_depth(S, orig));
     return 1.0f;
 }
 
@@ -303,12 +304,15 @@ SHD void refract_from(const DScene& S, const DHit& gh, v3 d, bool entering, floa
     o_out = gh.point + td * 0.002f;
 }
 
-SHD TStack make_stack(uint2* shared) {
-    TStack st;
-    st.lds = shared + threadIdx.x;
-    st.sp = 0;
-    return st;
-}
+// Every kernel that traverses declares its LDS columns and its private
+// spill array, then builds the stack view over them.
+#define SHEEN_STACK(st)                                   \
+    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];     \
+    uint2 s_spill[SPILL_DEPTH];                           \
+    TStack st;                                            \
+    st.lds = s_stack + threadIdx.x;                       \
+    st.spill = s_spill;                                   \
+    st.spills = 0
 
 // ---------------------------------------------------------------------------
 // Hit record of the wavefront buffers: {t bits, index, carried object id, kind}.
@@ -316,10 +320,9 @@ SHD int4 pack_hit(const QHit& q) {
     return make_int4(__float_as_int(q.t), q.idx, q.carry, q.kind);
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_primary(DScene S, FrameDesc F, int4* hit_p, Counters* C) {
-    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];
-    TStack st = make_stack(s_stack);
+template <bool COUNT, bool FULL>
+__global__ TRACE_BOUNDS void k_primary(DScene S, FrameDesc F, int4* hit_p, Counters* C) {
+    SHEEN_STACK(st);
     TraceStats ts{0, 0};
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < F.n_slots; i += stride) {
@@ -330,7 +333,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_primary(DScene S, FrameDesc F, 
             uint32_t seed;
             DRay r = camera_ray(S.cam, x, y, s, F.ns, F.width, F.height, pa, seed);
             QHit q;
-            query_closest<COUNT>(S, r, q, st, ts);
+            query_closest<COUNT, FULL>(S, r, q, st, ts);
             out = pack_hit(q);
         }
         hit_p[i] = out;
@@ -340,6 +343,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_primary(DScene S, FrameDesc F, 
         wave_add64(&C->prims, ts.prims);
         wave_add64(&C->pnodes, ts.nodes);
         wave_add64(&C->pprims, ts.prims);
+        wave_add64(&C->spills, st.spills);
     }
 }
 
@@ -352,10 +356,10 @@ SHD QHit unpack_hit(int4 h) {
     return q;
 }
 
-__global__ __launch_bounds__(TRACE_BLOCK) void k_shade_primary(DScene S, FrameDesc F, SampleBufs SB, EventBufs EB,
+template <bool FULL>
+__global__ SHADE_BOUNDS void k_shade_primary(DScene S, FrameDesc F, SampleBufs SB, EventBufs EB,
                                                                SecRay* sec_q, Counters* C) {
-    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];
-    TStack st = make_stack(s_stack);
+    SHEEN_STACK(st);
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t passes = 0;
     // loop bound rounded up to whole waves so every lane reaches the queue ballots
@@ -375,7 +379,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shade_primary(DScene S, FrameDe
         }
         const bool hit = valid && q.kind >= 0;
         DHit rec;
-        if (hit) rebuild_hit(S, r, q, rec);
+        if (hit) rebuild_hit<FULL>(S, r, q, rec);
         if (valid && !hit) sky_event(S, r.d, i * 3 + 0, EB);
         shade_event(S, hit, rec, -r.d, hit ? rec.t * pa : 0.0f, seed, i * 3 + 0, EB);
         float4 sp = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
@@ -409,12 +413,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shade_primary(DScene S, FrameDe
                 tr.meta = i | (1u << 28) | (1u << 29);
             }
         }
-        if (valid && SB.beam_add) beams_of(S, r, hit, hit ? rec.t : 0.0f, seed, SB.beam_add + (size_t)i * SB.n_beam, st, passes);
-        if (valid && SB.media_p) SB.media_p[i] = media_of(S, r, hit ? rec.t : 1.0e9f, seed, st, passes);
-        uint32_t a = wave_reserve(&C->sec_n[0], want_r);
+        if (FULL && valid && SB.beam_add)
+            beams_of(S, r, hit, hit ? rec.t : 0.0f, seed, SB.beam_add + (size_t)i * SB.n_beam, st, passes);
+        if (FULL && valid && SB.media_p) SB.media_p[i] = media_of(S, r, hit ? rec.t : 1.0e9f, seed, st, passes);
+        const uint32_t a = block_reserve(&C->sec_n[0], want_r, want_t);
         if (want_r) sec_q[a] = rr;
-        uint32_t b = wave_reserve(&C->sec_n[0], want_t);
-        if (want_t) sec_q[b] = tr;
+        if (want_t) sec_q[a + (want_r ? 1u : 0u)] = tr;
         if (i < F.n_slots) {
             sp.w = __int_as_float(flags);
             SB.samp[i] = sp;
@@ -423,11 +427,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shade_primary(DScene S, FrameDe
     wave_add64(&C->extra_shadow, passes);
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_sec(DScene S, const SecRay* q_in, int which, int4* hits,
+template <bool COUNT, bool FULL>
+__global__ TRACE_BOUNDS void k_trace_sec(DScene S, const SecRay* q_in, int which, int4* hits,
                                                           Counters* C) {
-    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];
-    TStack st = make_stack(s_stack);
+    SHEEN_STACK(st);
     TraceStats ts{0, 0};
     const uint32_t n = C->sec_n[which];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -440,7 +443,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_sec(DScene S, const SecRa
         r.tmin = 0.001f;
         r.tmax = 1.0e10f;
         QHit q;
-        query_closest<COUNT>(S, r, q, st, ts);
+        query_closest<COUNT, FULL>(S, r, q, st, ts);
         hits[j] = pack_hit(q);
         if (e.meta & (1u << 28)) nrefr++; else nrefl++;
     }
@@ -452,11 +455,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_sec(DScene S, const SecRa
     }
 }
 
-__global__ __launch_bounds__(TRACE_BLOCK) void k_shade_sec(DScene S, FrameDesc F, SampleBufs SB, EventBufs EB,
+template <bool FULL>
+__global__ SHADE_BOUNDS void k_shade_sec(DScene S, FrameDesc F, SampleBufs SB, EventBufs EB,
                                                           const SecRay* q_in, int which, const int4* hits,
                                                           SecRay* q_out, int which_out, Counters* C) {
-    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];
-    TStack st = make_stack(s_stack);
+    SHEEN_STACK(st);
     const uint32_t n = C->sec_n[which];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t n_iter = (n + stride - 1) / stride;
@@ -494,7 +497,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shade_sec(DScene S, FrameDesc F
         }
         const bool hit = valid && q.kind >= 0;
         DHit h;
-        if (hit) rebuild_hit(S, r, q, h);
+        if (hit) rebuild_hit<FULL>(S, r, q, h);
         bool lit = false;
         uint32_t ev = 0;
         bool want_next = false;
@@ -503,7 +506,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shade_sec(DScene S, FrameDesc F
             ev = i * 3 + 1;
             if (hit) lit = true;
             else sky_event(S, r.d, ev, EB);
-            if (SB.media_r) SB.media_r[i] = media_of(S, r, hit ? h.t : 1.0e9f, seed, st, passes);
+            if (FULL && SB.media_r) SB.media_r[i] = media_of(S, r, hit ? h.t : 1.0e9f, seed, st, passes);
         } else if (valid) {  // refraction face: render_kernel.cu:767-802
             ev = i * 3 + 2;
             const uint32_t face = e.meta >> 30;
@@ -533,29 +536,29 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shade_sec(DScene S, FrameDesc F
             }
         }
         shade_event(S, lit, h, -r.d, lit ? h.t * pa : 0.0f, seed, ev, EB);
-        uint32_t a = wave_reserve(&C->sec_n[which_out], want_next);
+        const uint32_t a = block_reserve(&C->sec_n[which_out], want_next, false);
         if (want_next) q_out[a] = nx;
     }
     wave_add64(&C->extra_shadow, passes);
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(DScene S, EventBufs EB, Counters* C) {
-    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];
-    TStack st = make_stack(s_stack);
+template <bool COUNT, bool FULL>
+__global__ TRACE_BOUNDS void k_shadow(DScene S, EventBufs EB, Counters* C) {
+    SHEEN_STACK(st);
     TraceStats ts{0, 0};
     const uint32_t n = C->shadow_n;
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t passes = 0;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         ShadowReq q = EB.shadow_q[j];
-        float tr = shadow_eval<COUNT>(S, q, st, passes, ts);
+        float tr = shadow_eval<COUNT, FULL>(S, q, st, passes, ts);
         EB.trans[q.slot & 0x7FFFFFFFu] = tr;
     }
     wave_add64(&C->shadow_passes, passes);
     if (COUNT) {
         wave_add64(&C->snodes, ts.nodes);
         wave_add64(&C->sprims, ts.prims);
+        wave_add64(&C->sspills, st.spills);
     }
 }
 

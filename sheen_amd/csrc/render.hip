@@ -75,6 +75,7 @@ struct SheenContext {
     DScene ds{};
     HostSceneInfo info;
     int n_beam = 0;
+    bool full = true;  // scene uses a kind beyond spheres, triangles and planes
     WorkBufs wb;
     float4* frame = nullptr;
     size_t frame_cap = 0;
@@ -82,6 +83,9 @@ struct SheenContext {
     Counters* ctr = nullptr;
     bool profiling = false, counting = false;
     long long chunk_override = 0;
+    // launch shapes: blocks per CU of each grid-stride kernel (0 = one work
+    // item per thread); queue-fed kernels are launched over the queue capacity
+    int primary_grid = 64, sec_grid = 6, shadow_grid = 96, shade_grid = 2, shade_sec_grid = 2;
     hipEvent_t ev_frame[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -127,8 +131,9 @@ struct SheenContext {
 
 static int grid_for(SheenContext* c, size_t n, int per_cu) {
     size_t blocks = (n + TRACE_BLOCK - 1) / TRACE_BLOCK;
-    size_t cap = (size_t)c->num_cu * per_cu;
+    size_t cap = per_cu > 0 ? (size_t)c->num_cu * per_cu : blocks;
     if (blocks > cap) blocks = cap;
+    if (blocks > 0x7FFFFFFF) blocks = 0x7FFFFFFF;
     if (blocks < 1) blocks = 1;
     return (int)blocks;
 }
@@ -303,6 +308,8 @@ static SheenContext* context_create(const SheenScene* in) {
         if (in->lights[i].type == 3 && in->lights[i].beam > 0.0f) c->n_beam++;
     d.any_beams = c->n_beam > 0;
     d.any_strands = d.count[C_STRAND] > 0;
+    c->full = d.count[C_STRAND] || d.count[C_HF] || d.count[C_SE] || d.count[C_LATHE] || d.count[C_PRISM] ||
+              d.csg_root >= 0 || d.num_media > 0 || c->n_beam > 0;
     if (!alloc_dev((void**)&c->ctr, sizeof(Counters))) { sheen_set_error("counter alloc failed"); return nullptr; }
     if (in->photons > 0 && d.num_lights > 0) {
         if (!photon_pass(c.get(), in)) return nullptr;
@@ -368,6 +375,61 @@ struct Timer {
     }
 };
 
+// The kernels of one frame chunk.  FULL = false selects the instantiation
+// compiled for spheres, triangles and planes only.
+template <bool FULL>
+static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, EventBufs& EB, SheenRayStats& S,
+                      bool hits_only, int* dk, int* di, float* dt) {
+    WorkBufs& w = c->wb;
+    hipStream_t s = c->stream;
+    const int ns = F.ns;
+        k_reset<<<1, 64, 0, s>>>(c->ctr);
+        {
+            Timer t(c, 0);
+            if (c->counting) k_primary<true, FULL><<<grid_for(c, F.n_slots, c->primary_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, c->ctr);
+            else k_primary<false, FULL><<<grid_for(c, F.n_slots, c->primary_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, c->ctr);
+            S.trace_launches++;
+        }
+        if (hits_only) {
+            k_export_hits<<<grid_for(c, F.n_slots, 8), 256, 0, s>>>(F, w.hit_p, dk, di, dt);
+            return;
+        }
+        {
+            Timer t(c, 4);
+            k_shade_primary<FULL><<<grid_for(c, F.n_slots, c->shade_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, SB, EB, w.sec_q[0], c->ctr);
+        }
+        // three rounds: reflection + first refraction face, then faces 2 and 3
+        for (int round = 0; round < 3; round++) {
+            const int in_q = round & 1, out_q = (round + 1) & 1;
+            const int in_ctr = round, out_ctr = round + 1;
+            const size_t cap = 2 * (size_t)F.n_slots;
+            {
+                Timer t(c, 1);
+                if (c->counting)
+                    k_trace_sec<true, FULL><<<grid_for(c, cap, c->sec_grid), TRACE_BLOCK, 0, s>>>(c->ds, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
+                else
+                    k_trace_sec<false, FULL><<<grid_for(c, cap, c->sec_grid), TRACE_BLOCK, 0, s>>>(c->ds, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
+                S.trace_launches++;
+            }
+            {
+                Timer t(c, 3);
+                k_shade_sec<FULL><<<grid_for(c, cap, c->shade_sec_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
+                                                                       w.sec_q[out_q], out_ctr, c->ctr);
+            }
+        }
+        {
+            Timer t(c, 2);
+            const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
+            if (c->counting) k_shadow<true, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
+            else k_shadow<false, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
+            S.shadow_launches++;
+        }
+        {
+            Timer t(c, 3);
+            k_accumulate<<<grid_for(c, F.n_slots / ns, 8), 256, 0, s>>>(c->ds, F, SB, EB, c->frame);
+        }
+}
+
 static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st, bool hits_only, int* hk, int* hi,
                         float* ht) {
     if (W <= 0 || H <= 0 || aa < 1 || aa > 64) { sheen_set_error("bad render params"); return -1; }
@@ -380,9 +442,9 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     size_t chunk;
     if (c->chunk_override > 0) chunk = (size_t)c->chunk_override;
     else {
-        const size_t budget = (size_t)3 << 30;  // 3 GiB of work buffers
+        const size_t budget = (size_t)8 << 30;  // 8 GiB of work buffers
         chunk = budget / slot_bytes(c->ds.num_lights, c->n_beam, caustic, c->ds.num_media > 0);
-        chunk = std::min(chunk, (size_t)8 << 20);
+        chunk = std::min(chunk, (size_t)16 << 20);
     }
     chunk = std::max(unit, chunk / unit * unit);
     chunk = std::min(chunk, (total + unit - 1) / unit * unit);
@@ -426,51 +488,8 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
         F.width = W; F.height = H; F.ns = ns; F.tiles_x = tiles_x;
         F.g0 = (uint32_t)g0;
         F.n_slots = (uint32_t)std::min(chunk, total - g0);
-        k_reset<<<1, 64, 0, s>>>(c->ctr);
-        {
-            Timer t(c, 0);
-            if (c->counting) k_primary<true><<<grid_for(c, F.n_slots, 16), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, c->ctr);
-            else k_primary<false><<<grid_for(c, F.n_slots, 16), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, c->ctr);
-            S.trace_launches++;
-        }
-        if (hits_only) {
-            k_export_hits<<<grid_for(c, F.n_slots, 8), 256, 0, s>>>(F, w.hit_p, dk, di, dt);
-            continue;
-        }
-        {
-            Timer t(c, 3);
-            k_shade_primary<<<grid_for(c, F.n_slots, 8), TRACE_BLOCK, 0, s>>>(c->ds, F, SB, EB, w.sec_q[0], c->ctr);
-        }
-        // three rounds: reflection + first refraction face, then faces 2 and 3
-        for (int round = 0; round < 3; round++) {
-            const int in_q = round & 1, out_q = (round + 1) & 1;
-            const int in_ctr = round, out_ctr = round + 1;
-            const size_t cap = 2 * (size_t)F.n_slots;
-            {
-                Timer t(c, 1);
-                if (c->counting)
-                    k_trace_sec<true><<<grid_for(c, cap, 16), TRACE_BLOCK, 0, s>>>(c->ds, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
-                else
-                    k_trace_sec<false><<<grid_for(c, cap, 16), TRACE_BLOCK, 0, s>>>(c->ds, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
-                S.trace_launches++;
-            }
-            {
-                Timer t(c, 3);
-                k_shade_sec<<<grid_for(c, cap, 8), TRACE_BLOCK, 0, s>>>(c->ds, F, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
-                                                                       w.sec_q[out_q], out_ctr, c->ctr);
-            }
-        }
-        {
-            Timer t(c, 2);
-            const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
-            if (c->counting) k_shadow<true><<<grid_for(c, cap, 16), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
-            else k_shadow<false><<<grid_for(c, cap, 16), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
-            S.shadow_launches++;
-        }
-        {
-            Timer t(c, 3);
-            k_accumulate<<<grid_for(c, F.n_slots / ns, 8), 256, 0, s>>>(c->ds, F, SB, EB, c->frame);
-        }
+        if (c->full) run_chunk<true>(c, F, SB, EB, S, hits_only, dk, di, dt);
+        else run_chunk<false>(c, F, SB, EB, S, hits_only, dk, di, dt);
     }
     HIP_OR(hipEventRecord(c->ev_frame[1], s), -1);
     HIP_OR(hipGetLastError(), -1);
@@ -484,6 +503,7 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
         if (m.slot == 0) { S.primary_ms += ms; S.trace_ms += ms; }
         else if (m.slot == 1) S.trace_ms += ms;
         else if (m.slot == 2) S.shadow_ms += ms;
+        else if (m.slot == 4) { S.shade_ms += ms; S.shade_primary_ms += ms; }
         else S.shade_ms += ms;
     }
     if (hits_only) {
@@ -504,6 +524,8 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     S.prim_tests = host.prims;
     S.primary_node_fetches = host.pnodes;
     S.primary_prim_tests = host.pprims;
+    S.primary_stack_spills = host.spills;
+    S.shadow_stack_spills = host.sspills;
     S.shadow_node_fetches = host.snodes;
     S.shadow_prim_tests = host.sprims;
     c->last = S;
@@ -594,6 +616,20 @@ int sheen_context_photon_probe(SheenContext* ctx, int stride, float radius, int 
     (void)hipFree(d);
     if (e != hipSuccess) { sheen_set_error("probe failed: %s", hipGetErrorString(e)); return -1; }
     return (int)probes;
+}
+
+int sheen_context_set_param(SheenContext* ctx, const char* name, int value) {
+    if (!ctx || !name) { sheen_set_error("null argument"); return -1; }
+    const std::string n(name);
+    if (value < 0) { sheen_set_error("negative parameter value"); return -1; }
+    if (n == "primary_grid") ctx->primary_grid = value;
+    else if (n == "sec_grid") ctx->sec_grid = value;
+    else if (n == "shadow_grid") ctx->shadow_grid = value;
+    else if (n == "shade_grid") ctx->shade_grid = value;
+    else if (n == "shade_sec_grid") ctx->shade_sec_grid = value;
+    else if (n == "chunk") ctx->chunk_override = value > 0 ? value : 0;
+    else { sheen_set_error("unknown parameter: %s", name); return -1; }
+    return 0;
 }
 
 int sheen_render_float(const SheenScene* scene, int width, int height, int aa_samples, float* out_rgba,

@@ -11,7 +11,7 @@ import numpy as np
 from . import contract
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-CAPI_PATH = os.path.join(HERE, "libsheen_c_api.so")
+CAPI_PATH = os.environ.get("SHEEN_CAPI_PATH") or os.path.join(HERE, "libsheen_c_api.so")
 HOST_PATH = os.path.join(HERE, "libsheen_host.so")
 
 
@@ -19,9 +19,9 @@ class SheenRayStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("primary_rays", "shadow_rays", "reflection_rays", "refraction_rays",
                                           "photon_rays", "total_rays", "samples", "node_fetches", "prim_tests",
                                           "shadow_node_fetches", "shadow_prim_tests", "primary_node_fetches",
-                                          "primary_prim_tests")] + \
+                                          "primary_prim_tests", "primary_stack_spills", "shadow_stack_spills")] + \
                [("frame_ms", C.c_float), ("trace_ms", C.c_float), ("primary_ms", C.c_float), ("shadow_ms", C.c_float), ("shade_ms", C.c_float),
-                ("trace_launches", C.c_int), ("shadow_launches", C.c_int)]
+                ("trace_launches", C.c_int), ("shadow_launches", C.c_int), ("shade_primary_ms", C.c_float)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -65,6 +65,7 @@ def capi():
     _sig(lib, "sheen_context_set_profiling", C.c_int, [P, C.c_int])
     _sig(lib, "sheen_context_set_counting", C.c_int, [P, C.c_int])
     _sig(lib, "sheen_context_set_chunk", C.c_int, [P, C.c_int])
+    _sig(lib, "sheen_context_set_param", C.c_int, [P, C.c_char_p, C.c_int])
     _sig(lib, "sheen_context_photon_stats", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)])
     _sig(lib, "sheen_context_photon_map", C.c_int, [P, P, P, P])
     _sig(lib, "sheen_context_photon_probe", C.c_int, [P, C.c_int, C.c_float, C.c_int, C.c_int, P])

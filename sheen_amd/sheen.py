@@ -78,6 +78,9 @@ class Context:
     def set_chunk(self, samples):
         self._lib.sheen_context_set_chunk(self._c, int(samples))
 
+    def set_param(self, name, value):
+        native.check(self._lib.sheen_context_set_param(self._c, name.encode(), int(value)))
+
     def photon_stats(self):
         e, s = C.c_int(), C.c_int()
         self._lib.sheen_context_photon_stats(self._c, C.byref(e), C.byref(s))

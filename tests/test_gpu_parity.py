@@ -82,6 +82,42 @@ def test_chunking_does_not_change_the_image():
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("cfg", [101, 105])
+def test_launch_shape_does_not_change_the_image(cfg):
+    """Queue order depends on the grid; every queue entry carries its sample
+    slot, so the image is bit-identical for any launch shape."""
+    from sheen_amd import scenes, sheen, autotune
+    sc = scenes.generate(cfg)
+    shapes = [{k: v for k, v in autotune.DEFAULTS.items()},
+              {"primary_grid": 1, "sec_grid": 1, "shadow_grid": 1, "shade_grid": 1, "shade_sec_grid": 1},
+              {"primary_grid": 0, "sec_grid": 0, "shadow_grid": 0, "shade_grid": 0, "shade_sec_grid": 0,
+               "chunk": 64 * 4 * 5}]
+    with sheen.Context(sc.scene) as ctx:
+        imgs = []
+        for p in shapes:
+            autotune.apply(ctx, p)
+            imgs.append(ctx.render(160, 120, 4)[0])
+    for img in imgs[1:]:
+        assert np.array_equal(img, imgs[0])
+
+
+def test_autotuner_picks_from_its_space_and_keeps_the_image():
+    from sheen_amd import scenes, sheen, autotune
+    sc = scenes.generate(2, detail=40)
+    space = {"shadow_grid": (8, 64), "shade_grid": (1, 4)}
+    with sheen.Context(sc.scene) as ctx:
+        ref, _ = ctx.render(192, 108, 4)
+        res = autotune.tune(ctx, 192, 108, 4, space=space, reps=1)
+        img, _ = ctx.render(192, 108, 4)
+        with pytest.raises(RuntimeError, match="unknown parameter"):
+            ctx.set_param("no_such_knob", 1)
+        with pytest.raises(RuntimeError, match="negative"):
+            ctx.set_param("shadow_grid", -1)
+    assert res["params"]["shadow_grid"] in (8, 64) and res["params"]["shade_grid"] in (1, 4)
+    assert len(res["trials"]) == 4 and res["frame_ms"] > 0
+    assert np.array_equal(img, ref)
+
+
 def test_photon_map_matches_oracle_and_reference_counts():
     """caustics.sheen: 10,868 of 20,000 photons stored (ROADMAP.md:313);
     hash-map gather equals brute force (ROADMAP.md:324-325).  Photon positions
