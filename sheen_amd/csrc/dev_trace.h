@@ -42,45 +42,37 @@
 #endif
 #define CULL_MARGIN 0.0078125f
 
-// 19-comparator sorting network for 8 keys (ascending).
-SHD void cswap(uint32_t& a, uint32_t& b) {
-    uint32_t lo = min(a, b), hi = max(a, b);
-    a = lo; b = hi;
-}
-SHD void sort8(uint32_t* k) {
-    cswap(k[0], k[2]); cswap(k[1], k[3]); cswap(k[4], k[6]); cswap(k[5], k[7]);
-    cswap(k[0], k[4]); cswap(k[1], k[5]); cswap(k[2], k[6]); cswap(k[3], k[7]);
-    cswap(k[0], k[1]); cswap(k[2], k[3]); cswap(k[4], k[5]); cswap(k[6], k[7]);
-    cswap(k[2], k[4]); cswap(k[3], k[5]);
-    cswap(k[1], k[4]); cswap(k[3], k[6]);
-    cswap(k[1], k[2]); cswap(k[3], k[4]); cswap(k[5], k[6]);
-}
-
 // Per-lane traversal stack: the top LDS_DEPTH entries live in LDS (one
 // column per lane, conflict-free), older entries spill to a private array
-// owned by the kernel.  The struct holds only pointers and counters so it
+// owned by the kernel.  An entry is a group of sibling nodes still to visit:
+// x = first internal child of their parent, y = bit mask of their ranks
+// (low 8 bits) | a lower bound of their entry distances (the float bits of
+// the nearest one with the low 8 bits cleared, which rounds toward zero).  The struct holds only pointers and counters so it
 // stays in registers; the stack pointer itself is a local of traverse().
 struct TStack {
     uint2* lds;    // &shared[tid], stride TRACE_BLOCK
     uint2* spill;  // private array of SPILL_DEPTH entries
     uint32_t spills;
 };
-SHD void stack_push(TStack& st, int& sp, uint32_t node, float t) {
-    const int slot = sp % LDS_DEPTH;
+SHD void stack_push(TStack& st, int& sp, uint2 e) {
+    const uint32_t slot = (uint32_t)sp % LDS_DEPTH;
     if (sp >= LDS_DEPTH) {
         st.spill[sp - LDS_DEPTH] = st.lds[slot * TRACE_BLOCK];
         st.spills++;
     }
-    st.lds[slot * TRACE_BLOCK] = make_uint2(node, __float_as_uint(t));
+    st.lds[slot * TRACE_BLOCK] = e;
     sp++;
 }
 SHD uint2 stack_pop(TStack& st, int& sp) {
     sp--;
-    const int slot = sp % LDS_DEPTH;
+    const uint32_t slot = (uint32_t)sp % LDS_DEPTH;
     uint2 e = st.lds[slot * TRACE_BLOCK];
     if (sp >= LDS_DEPTH) st.lds[slot * TRACE_BLOCK] = st.spill[sp - LDS_DEPTH];
     return e;
 }
+// The top entry (always in LDS) read and rewritten in place.
+SHD uint2 stack_top(const TStack& st, int sp) { return st.lds[(uint32_t)(sp - 1) % LDS_DEPTH * TRACE_BLOCK]; }
+SHD void stack_set_top(TStack& st, int sp, uint2 e) { st.lds[(uint32_t)(sp - 1) % LDS_DEPTH * TRACE_BLOCK] = e; }
 
 struct TraceStats {
     uint32_t nodes, prims;
@@ -128,93 +120,138 @@ SHD bool leaf_dist(const DScene& S, uint32_t slot, const DRay& r, v3 inv, float&
     return true;
 }
 
+// State of one in-flight traversal of one category's wide BVH: the node to
+// visit next, the stack depth, and the running best (ct: acceptance bound,
+// exclusive unless a same-category tie; ci/ch: best index / found).
+struct TravState {
+    uint32_t node;
+    int sp;
+    float ct;
+    int ci;
+    bool ch;
+};
+
+// CAT_MESH: the category (sphere or triangle) is chosen at run time.
+#define CAT_MESH (-1)
+
+template <int CAT>
+SHD bool leaf_test(const DScene& S, int cat, uint32_t slot, const DRay& r, v3 inv, float& t, int& idx) {
+    if constexpr (CAT == CAT_MESH) {
+        if (cat == C_TRI) return leaf_dist<C_TRI>(S, slot, r, inv, t, idx);
+        return leaf_dist<C_SPHERE>(S, slot, r, inv, t, idx);
+    } else {
+        return leaf_dist<CAT>(S, slot, r, inv, t, idx);
+    }
+}
+
+// One node visit: box tests of the eight children, the primitives of every
+// leaf child hit, then the next node (nearest internal child, else the
+// nearest stacked node still within the bound).  Returns true when the
+// traversal is over (stack exhausted, or with `any` the first accepted hit).
+template <int CAT, bool COUNT>
+SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, TravState& T, TStack& st,
+                   TraceStats& ts) {
+    const WNode* nodes;
+    if constexpr (CAT == CAT_MESH) nodes = (cat == C_TRI) ? S.bvh[C_TRI] : S.bvh[C_SPHERE];
+    else nodes = S.bvh[CAT];
+    const uint4* np = reinterpret_cast<const uint4*>(nodes + T.node);
+    uint4 h0 = np[0], h1 = np[1];
+    uint4 q0 = np[2], q1 = np[3], q2 = np[4], q3 = np[5], q4 = np[6], q5 = np[7];
+    if (COUNT) ts.nodes++;
+    const float ox = __uint_as_float(h0.x), oy = __uint_as_float(h0.y), oz = __uint_as_float(h0.z);
+    const float sx = __uint_as_float((h0.w & 0xFFu) << 23);
+    const float sy = __uint_as_float(((h0.w >> 8) & 0xFFu) << 23);
+    const float sz = __uint_as_float(((h0.w >> 16) & 0xFFu) << 23);
+    const uint32_t child_base = h1.x, prim_base = h1.y;
+    const uint64_t meta = (uint64_t)h1.z | ((uint64_t)h1.w << 32);
+    const float bound = T.ct + T.ct * CULL_MARGIN;
+    uint32_t keys[8];
+    uint32_t leafmask = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
+        const int w = i >> 1, sh = 16 * (i & 1);
+        auto g = [&](uint4 row) -> float {
+            uint32_t d = (w == 0) ? row.x : (w == 1) ? row.y : (w == 2) ? row.z : row.w;
+            return (float)((d >> sh) & 0xFFFFu);
+        };
+        const float lx = fmaf(g(q0), sx, ox), hx = fmaf(g(q1), sx, ox);
+        const float ly = fmaf(g(q2), sy, oy), hy = fmaf(g(q3), sy, oy);
+        const float lz = fmaf(g(q4), sz, oz), hz = fmaf(g(q5), sz, oz);
+        const float ax = (lx - r.o.x) * inv.x, bx = (hx - r.o.x) * inv.x;
+        const float ay = (ly - r.o.y) * inv.y, by = (hy - r.o.y) * inv.y;
+        const float az = (lz - r.o.z) * inv.z, bz = (hz - r.o.z) * inv.z;
+        const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), r.tmin));
+        const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), bound));
+        const bool hit = (m != WBVH_EMPTY) && (tn <= tf);
+        const bool leaf = (m >> 5) != 0;
+        keys[i] = (hit && !leaf) ? ((__float_as_uint(tn) & ~7u) | (m & 7u)) : 0xFFFFFFFFu;
+        if (hit && leaf) leafmask |= 1u << i;
+    }
+    while (leafmask) {
+        const int i = __builtin_ctz(leafmask);
+        leafmask &= leafmask - 1;
+        const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
+        const uint32_t cnt = m >> 5, off = m & 31u;
+        for (uint32_t k = 0; k < cnt; k++) {
+            float t;
+            int idx;
+            if (COUNT) ts.prims++;
+            if (leaf_test<CAT>(S, cat, prim_base + off + k, r, inv, t, idx)) {
+                if (t < T.ct || (t == T.ct && T.ch && idx < T.ci)) {
+                    T.ct = t;
+                    T.ci = idx;
+                    T.ch = true;
+                    if (any) return true;
+                }
+            }
+        }
+    }
+    // nearest internal child next; the other hit internal children become
+    // one stack entry bounded by the nearest of them
+    uint32_t k0 = 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < 8; i++) k0 = min(k0, keys[i]);
+    if (k0 != 0xFFFFFFFFu) {
+        uint32_t k1 = 0xFFFFFFFFu, rest = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const bool other = keys[i] != 0xFFFFFFFFu && keys[i] != k0;
+            k1 = min(k1, other ? keys[i] : 0xFFFFFFFFu);
+            rest |= other ? (1u << (keys[i] & 7u)) : 0u;
+        }
+        if (rest && T.sp < LDS_DEPTH + SPILL_DEPTH) stack_push(st, T.sp, make_uint2(child_base, (k1 & ~0xFFu) | rest));
+        T.node = child_base + (k0 & 7u);
+        return false;
+    }
+    const float lim = T.ct + T.ct * CULL_MARGIN;
+    while (T.sp > 0) {
+        const uint2 e = stack_top(st, T.sp);
+        if (__uint_as_float(e.y & ~0xFFu) > lim) {
+            (void)stack_pop(st, T.sp);
+            continue;
+        }
+        const uint32_t m = e.y & 0xFFu;
+        const uint32_t rest = m & (m - 1u);
+        if (rest) stack_set_top(st, T.sp, make_uint2(e.x, (e.y & ~0xFFu) | rest));
+        else (void)stack_pop(st, T.sp);
+        T.node = e.x + (uint32_t)__builtin_ctz(m);
+        return false;
+    }
+    return true;
+}
+
 // Closest (or, with ANY, first) hit of one category's wide BVH.
 // cat_t: in = acceptance bound (exclusive unless a same-category tie), out =
 // best distance; cat_idx/cat_hit as the running best of this category.
 template <int CAT, bool ANY, bool COUNT>
 SHD void traverse(const DScene& S, const DRay& r, v3 inv, float& cat_t, int& cat_idx, bool& cat_hit, TStack& st,
                   TraceStats& ts) {
-    const WNode* nodes = S.bvh[CAT];
-    int sp = 0;
-    uint32_t node = 0;
-    for (;;) {
-        const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
-        uint4 h0 = np[0], h1 = np[1];
-        uint4 q0 = np[2], q1 = np[3], q2 = np[4], q3 = np[5], q4 = np[6], q5 = np[7];
-        if (COUNT) ts.nodes++;
-        const float ox = __uint_as_float(h0.x), oy = __uint_as_float(h0.y), oz = __uint_as_float(h0.z);
-        const float sx = __uint_as_float((h0.w & 0xFFu) << 23);
-        const float sy = __uint_as_float(((h0.w >> 8) & 0xFFu) << 23);
-        const float sz = __uint_as_float(((h0.w >> 16) & 0xFFu) << 23);
-        const uint32_t child_base = h1.x, prim_base = h1.y;
-        const uint64_t meta = (uint64_t)h1.z | ((uint64_t)h1.w << 32);
-        const float bound = cat_t + cat_t * CULL_MARGIN;
-        uint32_t keys[8];
-        uint32_t leafmask = 0;
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
-            const int w = i >> 1, sh = 16 * (i & 1);
-            auto g = [&](uint4 row) -> float {
-                uint32_t d = (w == 0) ? row.x : (w == 1) ? row.y : (w == 2) ? row.z : row.w;
-                return (float)((d >> sh) & 0xFFFFu);
-            };
-            const float lx = fmaf(g(q0), sx, ox), hx = fmaf(g(q1), sx, ox);
-            const float ly = fmaf(g(q2), sy, oy), hy = fmaf(g(q3), sy, oy);
-            const float lz = fmaf(g(q4), sz, oz), hz = fmaf(g(q5), sz, oz);
-            const float ax = (lx - r.o.x) * inv.x, bx = (hx - r.o.x) * inv.x;
-            const float ay = (ly - r.o.y) * inv.y, by = (hy - r.o.y) * inv.y;
-            const float az = (lz - r.o.z) * inv.z, bz = (hz - r.o.z) * inv.z;
-            const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), r.tmin));
-            const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), bound));
-            const bool hit = (m != WBVH_EMPTY) && (tn <= tf);
-            const bool leaf = (m >> 5) != 0;
-            keys[i] = (hit && !leaf) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
-            if (hit && leaf) leafmask |= 1u << i;
-        }
-        while (leafmask) {
-            const int i = __builtin_ctz(leafmask);
-            leafmask &= leafmask - 1;
-            const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
-            const uint32_t cnt = m >> 5, off = m & 31u;
-            for (uint32_t k = 0; k < cnt; k++) {
-                float t;
-                int idx;
-                if (COUNT) ts.prims++;
-                if (leaf_dist<CAT>(S, prim_base + off + k, r, inv, t, idx)) {
-                    if (t < cat_t || (t == cat_t && cat_hit && idx < cat_idx)) {
-                        cat_t = t;
-                        cat_idx = idx;
-                        cat_hit = true;
-                        if (ANY) return;
-                    }
-                }
-            }
-        }
-        sort8(keys);
-        if (keys[0] != 0xFFFFFFFFu) {
-#pragma unroll
-            for (int j = 7; j >= 1; j--) {
-                if (keys[j] != 0xFFFFFFFFu) {
-                    const uint32_t m = (uint32_t)(meta >> (8 * (keys[j] & 7u))) & 0xFFu;
-                    if (sp < LDS_DEPTH + SPILL_DEPTH) stack_push(st, sp, child_base + (m & 31u), __uint_as_float(keys[j] & ~7u));
-                }
-            }
-            const uint32_t m0 = (uint32_t)(meta >> (8 * (keys[0] & 7u))) & 0xFFu;
-            node = child_base + (m0 & 31u);
-            continue;
-        }
-        bool found = false;
-        while (sp > 0) {
-            uint2 e = stack_pop(st, sp);
-            if (__uint_as_float(e.y) <= cat_t + cat_t * CULL_MARGIN) {
-                node = e.x;
-                found = true;
-                break;
-            }
-        }
-        if (!found) return;
+    TravState T;
+    T.node = 0; T.sp = 0; T.ct = cat_t; T.ci = cat_idx; T.ch = cat_hit;
+    while (!trav_step<CAT, COUNT>(S, CAT, ANY, r, inv, T, st, ts)) {
     }
+    cat_t = T.ct; cat_idx = T.ci; cat_hit = T.ch;
 }
 
 // CSG tree evaluation (reference integrator/csg.h:113-186) with an explicit

@@ -85,7 +85,7 @@ struct SheenContext {
     long long chunk_override = 0;
     // launch shapes: blocks per CU of each grid-stride kernel (0 = one work
     // item per thread); queue-fed kernels are launched over the queue capacity
-    int primary_grid = 64, sec_grid = 6, shadow_grid = 96, shade_grid = 2, shade_sec_grid = 2;
+    int primary_grid = 64, sec_grid = 6, shadow_grid = 64, shade_grid = 2, shade_sec_grid = 2;
     hipEvent_t ev_frame[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -420,8 +420,9 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
         {
             Timer t(c, 2);
             const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
-            if (c->counting) k_shadow<true, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
-            else k_shadow<false, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
+            const int g = grid_for(c, cap, c->shadow_grid);
+            if (c->counting) k_shadow<true, FULL><<<g, TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
+            else k_shadow<false, FULL><<<g, TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
             S.shadow_launches++;
         }
         {

@@ -273,7 +273,9 @@ void wbvh_build(const float* bmin, const float* bmax, int n, const unsigned char
         if ((int)n_internal.size() <= it.wide) n_internal.resize(it.wide + 1, 0);
         n_internal[it.wide] = ni;
     }
-    // Worst-case stack entries: need(v) = (internal(v) - 1)^+ + max need(child).
+    // Worst-case stack entries: a node with two or more internal children
+    // keeps one entry (its unvisited siblings) while a child is explored:
+    // need(v) = [internal(v) >= 2] + max need(child).
     int nn = (int)out.nodes.size();
     n_internal.resize(nn, 0);
     std::vector<int> need(nn, 0);
@@ -285,7 +287,7 @@ void wbvh_build(const float* bmin, const float* bmax, int n, const unsigned char
             if (mt == WBVH_EMPTY || (mt >> 5) != 0) continue;
             best = std::max(best, need[w.child_base + (mt & 31)]);
         }
-        need[v] = std::max(0, n_internal[v] - 1) + best;
+        need[v] = (n_internal[v] >= 2 ? 1 : 0) + best;
     }
     out.max_stack = need[0];
     for (int l : level) out.depth = std::max(out.depth, l);

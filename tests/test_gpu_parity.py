@@ -82,7 +82,7 @@ def test_chunking_does_not_change_the_image():
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("cfg", [101, 105])
+@pytest.mark.parametrize("cfg", [1, 101, 102, 105])
 def test_launch_shape_does_not_change_the_image(cfg):
     """Queue order depends on the grid; every queue entry carries its sample
     slot, so the image is bit-identical for any launch shape."""
