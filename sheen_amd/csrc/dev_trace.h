@@ -40,6 +40,9 @@
 #else
 #define SHADE_BOUNDS __launch_bounds__(TRACE_BLOCK)
 #endif
+#ifndef BOX_FAST
+#define BOX_FAST 1
+#endif
 #define CULL_MARGIN 0.0078125f
 
 // Per-lane traversal stack: the top LDS_DEPTH entries live in LDS (one
@@ -167,26 +170,64 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
     const float bound = T.ct + T.ct * CULL_MARGIN;
     uint32_t keys[8];
     uint32_t leafmask = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
+    auto g = [](uint4 row, int i) -> float {
         const int w = i >> 1, sh = 16 * (i & 1);
-        auto g = [&](uint4 row) -> float {
-            uint32_t d = (w == 0) ? row.x : (w == 1) ? row.y : (w == 2) ? row.z : row.w;
-            return (float)((d >> sh) & 0xFFFFu);
-        };
-        const float lx = fmaf(g(q0), sx, ox), hx = fmaf(g(q1), sx, ox);
-        const float ly = fmaf(g(q2), sy, oy), hy = fmaf(g(q3), sy, oy);
-        const float lz = fmaf(g(q4), sz, oz), hz = fmaf(g(q5), sz, oz);
-        const float ax = (lx - r.o.x) * inv.x, bx = (hx - r.o.x) * inv.x;
-        const float ay = (ly - r.o.y) * inv.y, by = (hy - r.o.y) * inv.y;
-        const float az = (lz - r.o.z) * inv.z, bz = (hz - r.o.z) * inv.z;
-        const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), r.tmin));
-        const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), bound));
-        const bool hit = (m != WBVH_EMPTY) && (tn <= tf);
-        const bool leaf = (m >> 5) != 0;
-        keys[i] = (hit && !leaf) ? ((__float_as_uint(tn) & ~7u) | (m & 7u)) : 0xFFFFFFFFu;
-        if (hit && leaf) leafmask |= 1u << i;
+        const uint32_t d = (w == 0) ? row.x : (w == 1) ? row.y : (w == 2) ? row.z : row.w;
+        return (float)((d >> sh) & 0xFFFFu);
+    };
+    // Distance-space decode: a plane q of axis a lies at t = q*B_a + A_a with
+    // B_a = s_a*inv_a (exact: s_a is a power of two) and A_a = (o_a - ray_a)*inv_a.
+    // Rounding makes this differ from the exact-path value ((q*s+o) - ray)*inv
+    // by less than E_a = 2^-20 (|A_a| + 65535|B_a| + (|o_a| + 65535 s_a)|inv_a|)
+    // (the terms need about 7u|A| + 4u*65535|B| + u(|o|+65535s)|inv|, u = 2^-24),
+    // so near planes offset by -E_a and far planes by +E_a bound the exact-path
+    // interval from outside and the test stays a superset.  The offset is per
+    // axis: an axis the ray nearly parallels has huge |inv| and planes far off
+    // in t, and its E must not widen the other axes.  A node whose E is not
+    // finite (a zero direction component, overflow) takes the exact path for
+    // the whole wave.
+    const float Ax = (ox - r.o.x) * inv.x, Ay = (oy - r.o.y) * inv.y, Az = (oz - r.o.z) * inv.z;
+    const float Bx = sx * inv.x, By = sy * inv.y, Bz = sz * inv.z;
+    const float Ex = fmaf(fmaf(fmaf(65535.0f, sx, fabsf(ox)), fabsf(inv.x), fmaf(65535.0f, fabsf(Bx), fabsf(Ax))), 0x1p-20f, 0x1p-100f);
+    const float Ey = fmaf(fmaf(fmaf(65535.0f, sy, fabsf(oy)), fabsf(inv.y), fmaf(65535.0f, fabsf(By), fabsf(Ay))), 0x1p-20f, 0x1p-100f);
+    const float Ez = fmaf(fmaf(fmaf(65535.0f, sz, fabsf(oz)), fabsf(inv.z), fmaf(65535.0f, fabsf(Bz), fabsf(Az))), 0x1p-20f, 0x1p-100f);
+    const float inf = __builtin_huge_valf();
+    const bool finite = Ex < inf && Ey < inf && Ez < inf;  // false for NaN too
+    if (BOX_FAST && __ballot(!finite) == 0ull) {
+        const bool px = inv.x >= 0.0f, py = inv.y >= 0.0f, pz = inv.z >= 0.0f;
+        const uint4 nx = px ? q0 : q1, fx = px ? q1 : q0;
+        const uint4 ny = py ? q2 : q3, fy = py ? q3 : q2;
+        const uint4 nz = pz ? q4 : q5, fz = pz ? q5 : q4;
+        const float Anx = Ax - Ex, Afx = Ax + Ex, Any = Ay - Ey, Afy = Ay + Ey, Anz = Az - Ez, Afz = Az + Ez;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
+            const float tn = fmaxf(fmaxf(fmaxf(fmaf(g(nx, i), Bx, Anx), fmaf(g(ny, i), By, Any)), fmaf(g(nz, i), Bz, Anz)),
+                                   r.tmin);
+            const float tf = fminf(fminf(fminf(fmaf(g(fx, i), Bx, Afx), fmaf(g(fy, i), By, Afy)), fmaf(g(fz, i), Bz, Afz)),
+                                   bound);
+            const bool hit = (m != WBVH_EMPTY) && (tn <= tf);
+            const bool leaf = (m >> 5) != 0;
+            keys[i] = (hit && !leaf) ? ((__float_as_uint(tn) & ~7u) | (m & 7u)) : 0xFFFFFFFFu;
+            if (hit && leaf) leafmask |= 1u << i;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
+            const float lx = fmaf(g(q0, i), sx, ox), hx = fmaf(g(q1, i), sx, ox);
+            const float ly = fmaf(g(q2, i), sy, oy), hy = fmaf(g(q3, i), sy, oy);
+            const float lz = fmaf(g(q4, i), sz, oz), hz = fmaf(g(q5, i), sz, oz);
+            const float ax = (lx - r.o.x) * inv.x, bx = (hx - r.o.x) * inv.x;
+            const float ay = (ly - r.o.y) * inv.y, by = (hy - r.o.y) * inv.y;
+            const float az = (lz - r.o.z) * inv.z, bz = (hz - r.o.z) * inv.z;
+            const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), r.tmin));
+            const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), bound));
+            const bool hit = (m != WBVH_EMPTY) && (tn <= tf);
+            const bool leaf = (m >> 5) != 0;
+            keys[i] = (hit && !leaf) ? ((__float_as_uint(tn) & ~7u) | (m & 7u)) : 0xFFFFFFFFu;
+            if (hit && leaf) leafmask |= 1u << i;
+        }
     }
     while (leafmask) {
         const int i = __builtin_ctz(leafmask);

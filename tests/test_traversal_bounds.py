@@ -1,0 +1,79 @@
+"""The distance-space child-box test of the wide-BVH traversal must contain
+the exact-path interval of every child (so the node test stays a superset
+of the reference slab test on primitive bounds).
+
+Float32 emulation: + - * are IEEE-rounded by numpy; fmaf(q, B, A) with a
+16-bit q is emulated as an exact float64 product plus a float64 sum rounded
+to float32 (double rounding can move a result by one ulp in rare halfway
+cases, far inside the 16x margin of the bound).
+"""
+import numpy as np
+
+F = np.float32
+
+
+def fma32(a, b, c):
+    return (a.astype(np.float64) * b.astype(np.float64) + c.astype(np.float64)).astype(F)
+
+
+def exact_path(q_lo, q_hi, o, s, ro, inv):
+    lo = fma32(q_lo, s, o)
+    hi = fma32(q_hi, s, o)
+    a = ((lo - ro).astype(F) * inv).astype(F)
+    b = ((hi - ro).astype(F) * inv).astype(F)
+    return np.minimum(a, b), np.maximum(a, b)
+
+
+def distance_space(q_lo, q_hi, o, s, ro, inv):
+    A = ((o - ro).astype(F) * inv).astype(F)
+    B = (s * inv).astype(F)
+    e = fma32(fma32(F(65535.0) * np.ones_like(s), s, np.abs(o)), np.abs(inv),
+              fma32(F(65535.0) * np.ones_like(B), np.abs(B), np.abs(A)))
+    E = fma32(e, F(2.0 ** -20) * np.ones_like(e), F(2.0 ** -100) * np.ones_like(e))
+    pos = inv >= 0
+    qn = np.where(pos, q_lo, q_hi)
+    qf = np.where(pos, q_hi, q_lo)
+    near = fma32(qn, B, (A - E).astype(F))
+    far = fma32(qf, B, (A + E).astype(F))
+    return near, far
+
+
+def _case(rng, n, scale):
+    o = (rng.uniform(-1, 1, n) * scale).astype(F)
+    exp = rng.integers(-30, 12, n)
+    s = np.ldexp(np.ones(n), exp).astype(F)
+    q_lo = rng.integers(0, 65535, n).astype(F)
+    q_hi = np.minimum(q_lo + rng.integers(0, 65535, n), 65535).astype(F)
+    d = rng.normal(size=n).astype(F)
+    d = np.where(rng.random(n) < 0.1, (d * 1e-6).astype(F), d)
+    inv = (F(1.0) / d).astype(F)
+    # ray origins: anywhere, or within a few ulps of a decoded box plane
+    ro = (o + rng.normal(size=n).astype(F) * F(scale)).astype(F)
+    plane = fma32(q_lo, s, o)
+    near_plane = rng.random(n) < 0.3
+    ro = np.where(near_plane, (plane + (rng.integers(-3, 4, n) * np.spacing(plane))).astype(F), ro)
+    return q_lo, q_hi, o, s, ro, inv
+
+
+def test_distance_space_interval_contains_exact_interval():
+    rng = np.random.default_rng(7)
+    for scale in (1e-3, 1.0, 30.0, 1e4, 1e7):
+        args = _case(rng, 200000, scale)
+        en, ef = exact_path(*args)
+        fn, ff = distance_space(*args)
+        ok = np.isfinite(en) & np.isfinite(ef) & np.isfinite(fn) & np.isfinite(ff)
+        assert ok.mean() > 0.99
+        assert np.all(fn[ok] <= en[ok]), scale
+        assert np.all(ff[ok] >= ef[ok]), scale
+
+
+def test_margin_is_needed():
+    """Without the E offsets the distance-space planes fall on both sides of
+    the exact-path planes, so the margin is load-bearing."""
+    rng = np.random.default_rng(11)
+    q_lo, q_hi, o, s, ro, inv = _case(rng, 200000, 30.0)
+    en, _ = exact_path(q_lo, q_hi, o, s, ro, inv)
+    A = ((o - ro).astype(F) * inv).astype(F)
+    B = (s * inv).astype(F)
+    near = fma32(np.where(inv >= 0, q_lo, q_hi), B, A)
+    assert np.any(near > en) and np.any(near < en)
