@@ -185,6 +185,37 @@ int sheen_context_set_chunk(SheenContext* ctx, int samples_per_chunk);
  * value. */
 int sheen_context_set_param(SheenContext* ctx, const char* name, int value);
 
+/* ---- Differentiable shading ------------------------------------------------
+ * Forward/backward entries for gradient-based inverse rendering (the
+ * reference has none; its ABI lists them as additions a replacement should
+ * provide).  Parameters, all float32 device arrays:
+ *   albedo     [materials * 3]  Material.diffuse_color
+ *   emission   [materials * 3]  emitted radiance, emission_color * emission
+ *   intensity  [lights]         Light.intensity
+ *   texels     [texels]         level-0 texels of every texture in texture
+ *                               order, RGB in [0, 1] (8-bit value / 255)
+ * The forward renders the context's scene with those values (textures as
+ * float chains whose mips are 2x2 means); the image is float RGBA
+ * [height * width * 4] in device memory.  The backward replays the same frame
+ * (same RNG streams) and returns d(sum grad_image * image)/d(parameter) with
+ * visibility held fixed.  Scenes with reflection, transmission, hair, debug
+ * materials, caustics, media, beams or primitives other than spheres,
+ * triangles and planes are refused (-1, reason in sheen_get_last_error).
+ * `stream` is a hipStream_t (NULL = the null stream); work is ordered after
+ * earlier work on it and later work on it waits for the results.  The
+ * context's own materials, lights and textures are restored after each call.
+ */
+int sheen_adjoint_sizes(SheenContext* ctx, int* materials, int* lights, long long* texels);
+int sheen_adjoint_get_params(SheenContext* ctx, float* albedo, float* emission, float* intensity, float* texels,
+                             void* stream);
+int sheen_adjoint_forward(SheenContext* ctx, int width, int height, int aa_samples, const float* albedo,
+                          const float* emission, const float* intensity, const float* texels, float* image,
+                          void* stream);
+int sheen_adjoint_backward(SheenContext* ctx, int width, int height, int aa_samples, const float* albedo,
+                           const float* emission, const float* intensity, const float* texels,
+                           const float* grad_image, float* d_albedo, float* d_emission, float* d_intensity,
+                           float* d_texels, void* stream);
+
 /* Photon pass results of a context (scene.photons > 0): photons emitted and
  * stored (reference render_kernel.cu:928-953). */
 int sheen_context_photon_stats(SheenContext* ctx, int* emitted, int* stored);

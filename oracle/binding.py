@@ -60,6 +60,7 @@ def lib():
     L.oracle_photon_probe.argtypes = [P, C.c_int, C.c_float, C.c_int, P]
     L.oracle_photon_probe_brute.restype = C.c_int
     L.oracle_photon_probe_brute.argtypes = [P, C.c_int, C.c_float, C.c_int, P]
+    L.oracle_set_params.argtypes = [P, P, P, P, P]
     _lib = L
     return L
 
@@ -133,6 +134,15 @@ class Oracle:
         f = self._L.oracle_photon_probe_brute if brute else self._L.oracle_photon_probe
         k = f(self._h, stride, radius, max_samples, _p(out))
         return out[:k]
+
+    def set_params(self, albedo=None, emission=None, intensity=None, texels=None):
+        """Substitutes the differentiable parameters (float32 arrays: albedo and
+        emission (materials, 3), intensity (lights,), texels (level 0 of every
+        texture, RGB / 255))."""
+        import numpy as np
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (albedo, emission, intensity, texels)]
+        ptrs = [None if a is None else a.ctypes.data_as(C.c_void_p) for a in keep]
+        self._L.oracle_set_params(self._h, *ptrs)
 
     def close(self):
         if self._h:

@@ -136,7 +136,39 @@ template <class R> struct Hit {
 // ---------------------------------------------------------------------------
 // Scene copy (host) with category BVHs
 // ---------------------------------------------------------------------------
-struct Tex { std::vector<unsigned char> px; int w = 0, h = 0, mips = 1; bool valid = false; };
+// fpx/dpx: float texel chains set by oracle_set_params (level 0 given, mips
+// as 2x2 means ((a + b) + (c + d)) * 0.25 in float and in double); when
+// use_float is set they replace the 8-bit chain.
+struct Tex {
+    std::vector<unsigned char> px;
+    std::vector<float> fpx;
+    std::vector<double> dpx;
+    int w = 0, h = 0, mips = 1;
+    bool valid = false, use_float = false;
+};
+template <class R> inline const R* float_chain(const Tex& t);
+template <> inline const float* float_chain<float>(const Tex& t) { return t.fpx.data(); }
+template <> inline const double* float_chain<double>(const Tex& t) { return t.dpx.data(); }
+template <class R> inline void build_float_mips(std::vector<R>& c, int w, int h) {
+    size_t src = 0;
+    int sw = w, sh = h;
+    while (sw > 1 || sh > 1) {
+        const int nw = sw > 1 ? sw / 2 : 1, nh = sh > 1 ? sh / 2 : 1;
+        const size_t dst = src + (size_t)sw * sh * 3;
+        for (int y = 0; y < nh; y++) {
+            const int ya = std::min(2 * y, sh - 1), yb = std::min(2 * y + 1, sh - 1);
+            for (int x = 0; x < nw; x++) {
+                const int xa = std::min(2 * x, sw - 1), xb = std::min(2 * x + 1, sw - 1);
+                for (int k = 0; k < 3; k++) {
+                    const R a = c[src + ((size_t)ya * sw + xa) * 3 + k], b = c[src + ((size_t)ya * sw + xb) * 3 + k];
+                    const R d = c[src + ((size_t)yb * sw + xa) * 3 + k], e = c[src + ((size_t)yb * sw + xb) * 3 + k];
+                    c[dst + ((size_t)y * nw + x) * 3 + k] = ((a + b) + (d + e)) * R(0.25);
+                }
+            }
+        }
+        src = dst; sw = nw; sh = nh;
+    }
+}
 
 // reference scene.h:230-261 (2x2 box, (s+2)/4, levels contiguous)
 inline void build_mips(Tex& t) {
@@ -182,6 +214,10 @@ struct Photons {
 struct Scene {
     SheenScene s{};
     std::vector<Tex> tex;
+    std::vector<Material> own_mats;  // set by oracle_set_params
+    std::vector<Light> own_lights;
+    const Material* orig_mats = nullptr;  // the caller's arrays
+    const Light* orig_lights = nullptr;
     std::vector<BNode> bvh[7];  // sphere, strand, tri, hf, se, lathe, prism
     int count[7] = {0, 0, 0, 0, 0, 0, 0};
     Photons pm;
@@ -1130,6 +1166,7 @@ template <class R> inline V3<R> sample_texture_lod(const Tex& t, R uvx, R uvy, R
         h = (h > 1) ? h / 2 : 1;
     }
     const unsigned char* base = t.px.data() + off;
+    const R* fbase = t.use_float ? float_chain<R>(t) + off : nullptr;
     R u = uvx - std::floor(uvx);
     R v = R(1.0f) - (uvy - std::floor(uvy));
     R fx = u * R(w - 1);
@@ -1139,6 +1176,10 @@ template <class R> inline V3<R> sample_texture_lod(const Tex& t, R uvx, R uvy, R
     int y1 = (y0 + 1) % h;
     R tx = fx - R(x0), ty = fy - R(y0);
     auto at = [&](int x, int y) {
+        if (fbase) {
+            const R* q = fbase + (size_t)(y * w + x) * 3;
+            return mk<R>(q[0], q[1], q[2]);
+        }
         const unsigned char* p = base + (size_t)(y * w + x) * 3;
         return mk<R>(R(p[0]) / R(255.0f), R(p[1]) / R(255.0f), R(p[2]) / R(255.0f));
     };
@@ -1823,6 +1864,43 @@ void* oracle_scene_create(const SheenScene* in, int threads) {
 }
 
 void oracle_scene_destroy(void* h) { delete (Scene*)h; }
+
+// Parameter substitution for gradient checks (the differentiable entry
+// points' forward): albedo -> diffuse_color, emission -> emission_color with
+// emission 1, intensity -> Light.intensity, texels (nullable) -> float chains.
+int oracle_set_params(void* h, const float* albedo, const float* emission, const float* intensity, const float* texels) {
+    Scene* S = (Scene*)h;
+    SheenScene& s = S->s;
+    if (!S->orig_mats) { S->orig_mats = s.materials; S->orig_lights = s.lights; }
+    S->own_mats.assign(S->orig_mats, S->orig_mats + s.num_materials);
+    for (int i = 0; i < s.num_materials; i++) {
+        Material& m = S->own_mats[i];
+        if (albedo) m.diffuse_color = {albedo[3 * i], albedo[3 * i + 1], albedo[3 * i + 2]};
+        if (emission) { m.emission_color = {emission[3 * i], emission[3 * i + 1], emission[3 * i + 2]}; m.emission = 1.0f; }
+    }
+    s.materials = S->own_mats.data();
+    if (s.lights && s.num_lights > 0) {
+        S->own_lights.assign(S->orig_lights, S->orig_lights + s.num_lights);
+        for (int i = 0; i < s.num_lights; i++)
+            if (intensity) S->own_lights[i].intensity = intensity[i];
+        s.lights = S->own_lights.data();
+    }
+    if (texels) {
+        size_t p = 0;
+        for (Tex& t : S->tex) {
+            if (!t.valid) continue;
+            const size_t n0 = (size_t)t.w * t.h * 3;
+            t.fpx.assign(t.px.size(), 0.0f);
+            t.dpx.assign(t.px.size(), 0.0);
+            for (size_t k = 0; k < n0; k++) { t.fpx[k] = texels[p + k]; t.dpx[k] = texels[p + k]; }
+            build_float_mips(t.fpx, t.w, t.h);
+            build_float_mips(t.dpx, t.w, t.h);
+            t.use_float = true;
+            p += n0;
+        }
+    }
+    return 0;
+}
 
 int oracle_photon_stats(void* h, int* emitted, int* stored) {
     Scene* S = (Scene*)h;

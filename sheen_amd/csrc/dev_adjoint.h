@@ -1,0 +1,236 @@
+// dev_adjoint.h — differentiable shading of primary hits.
+//
+// Parameters: per-material albedo (diffuse_color) and emitted radiance
+// (emission_color * emission), per-light intensity, and the level-0 texels of
+// every texture as floats.  The forward is the ordinary wavefront render with
+// those values substituted (float texel chains whose mips are 2x2 means).
+// The adjoint replays the frame (primary trace, shading, shadow rays with the
+// same RNG streams) and differentiates the radiance of each primary hit with
+// visibility held fixed: for fixed geometry and shadow transmittance the
+// pixel is linear in every parameter group.  Per sample (render_kernel.cu:
+// 26-298 restricted to surfaces without reflection, refraction, hair and
+// caustics):
+//   c = base * ambient + E + sum_l tr_l * (base * rad_l * diff_l + rad_l * spec * s_l)
+// with base = albedo or the bilinear texture tap, rad_l = intensity_l * q_l.
+#pragma once
+#include "kernels.h"
+
+struct AdjGrads {
+    float* albedo;     // [materials * 3]
+    float* emission;   // [materials * 3]
+    float* intensity;  // [lights]
+    float* tex;        // gradient of every float texel chain (tex_f layout)
+};
+
+// Writes a parameter set into the device materials and lights.
+__global__ void k_adj_set(Material* mats, int M, Light* lights, int L, const float* albedo, const float* emission,
+                          const float* intensity) {
+    const int stride = gridDim.x * blockDim.x;
+    const int n = M > L ? M : L;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (i < M) {
+            mats[i].diffuse_color = {albedo[3 * i], albedo[3 * i + 1], albedo[3 * i + 2]};
+            mats[i].emission_color = {emission[3 * i], emission[3 * i + 1], emission[3 * i + 2]};
+            mats[i].emission = 1.0f;
+        }
+        if (i < L) lights[i].intensity = intensity[i];
+    }
+}
+
+// The scene's own parameter values.
+__global__ void k_adj_get(const Material* mats, int M, const Light* lights, int L, float* albedo, float* emission,
+                          float* intensity) {
+    const int stride = gridDim.x * blockDim.x;
+    const int n = M > L ? M : L;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (i < M) {
+            const Material& m = mats[i];
+            albedo[3 * i] = m.diffuse_color.x; albedo[3 * i + 1] = m.diffuse_color.y; albedo[3 * i + 2] = m.diffuse_color.z;
+            emission[3 * i] = m.emission_color.x * m.emission;
+            emission[3 * i + 1] = m.emission_color.y * m.emission;
+            emission[3 * i + 2] = m.emission_color.z * m.emission;
+        }
+        if (i < L) intensity[i] = lights[i].intensity;
+    }
+}
+
+__global__ void k_bytes_to_float(const unsigned char* px, uint32_t n, float* out) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = px[i] / 255.0f;
+}
+
+__global__ void k_fill_f(float* p, size_t n, float v) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// One float mip level from the one below (the 8-bit chain's 2x2 rule with
+// clamped odd edges, as a mean instead of rounded integers):
+// parent = ((a + b) + (c + d)) * 0.25.
+__global__ void k_mip_f(float* chain, uint32_t src, int sw, int sh, uint32_t dst, int nw, int nh) {
+    const int n = nw * nh;
+    const int stride = gridDim.x * blockDim.x;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        const int y = p / nw, x = p - y * nw;
+        const int ya = min(2 * y, sh - 1), yb = min(2 * y + 1, sh - 1);
+        const int xa = min(2 * x, sw - 1), xb = min(2 * x + 1, sw - 1);
+        for (int c = 0; c < 3; c++) {
+            const float a = chain[src + (uint32_t)(ya * sw + xa) * 3u + c], b = chain[src + (uint32_t)(ya * sw + xb) * 3u + c];
+            const float d = chain[src + (uint32_t)(yb * sw + xa) * 3u + c], e = chain[src + (uint32_t)(yb * sw + xb) * 3u + c];
+            chain[dst + (uint32_t)p * 3u + c] = ((a + b) + (d + e)) * 0.25f;
+        }
+    }
+}
+
+// Adjoint of k_mip_f: each parent hands a quarter of its gradient to each of
+// its four (clamped) children.  For a level at least 2 wide and tall every
+// child has one parent; a 1-wide level repeats its column (the child gets
+// both quarters), so plain adds are race free.
+__global__ void k_mip_adj(float* g, uint32_t src, int sw, int sh, uint32_t dst, int nw, int nh) {
+    const int n = nw * nh;
+    const int stride = gridDim.x * blockDim.x;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        const int y = p / nw, x = p - y * nw;
+        const int ya = min(2 * y, sh - 1), yb = min(2 * y + 1, sh - 1);
+        const int xa = min(2 * x, sw - 1), xb = min(2 * x + 1, sw - 1);
+        for (int c = 0; c < 3; c++) {
+            const float q = g[dst + (uint32_t)p * 3u + c] * 0.25f;
+            g[src + (uint32_t)(ya * sw + xa) * 3u + c] += q;
+            g[src + (uint32_t)(ya * sw + xb) * 3u + c] += q;
+            g[src + (uint32_t)(yb * sw + xa) * 3u + c] += q;
+            g[src + (uint32_t)(yb * sw + xb) * 3u + c] += q;
+        }
+    }
+}
+
+SHD float wave_sum_f(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Adds `val` (3 floats) to dst[3*key] for every lane with key >= 0, one atomic
+// per distinct key per wave.  Every lane of the wave must call it.
+SHD void wave_add_keyed(float* dst, int key, v3 val) {
+    uint64_t pending = __ballot(key >= 0);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int k = __shfl(key, leader);
+        const bool mine = key == k;
+        const float x = wave_sum_f(mine ? val.x : 0.0f);
+        const float y = wave_sum_f(mine ? val.y : 0.0f);
+        const float z = wave_sum_f(mine ? val.z : 0.0f);
+        if ((int)__lane_id() == leader) {
+            unsafeAtomicAdd(dst + 3 * k, x);
+            unsafeAtomicAdd(dst + 3 * k + 1, y);
+            unsafeAtomicAdd(dst + 3 * k + 2, z);
+        }
+        pending &= ~__ballot(mine);
+    }
+}
+
+// Adds val (3 floats) at dst[e .. e+2] for active lanes.  Lanes of a wave that
+// share an address (a coarse mip level seen by many pixels) are summed first,
+// for up to four distinct addresses, so a hot texel takes one atomic per wave
+// instead of one per lane; the other lanes add directly.  Every lane of the
+// wave must call it.
+SHD void wave_add_shared(float* dst, bool active, uint32_t e, v3 val) {
+    uint64_t pending = __ballot(active);
+    for (int round = 0; round < 4 && pending; round++) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t k = __shfl(e, leader);
+        const bool mine = active && ((pending >> __lane_id()) & 1ull) && e == k;
+        const float x = wave_sum_f(mine ? val.x : 0.0f);
+        const float y = wave_sum_f(mine ? val.y : 0.0f);
+        const float z = wave_sum_f(mine ? val.z : 0.0f);
+        if ((int)__lane_id() == leader) {
+            if (x != 0.0f) unsafeAtomicAdd(dst + k, x);
+            if (y != 0.0f) unsafeAtomicAdd(dst + k + 1, y);
+            if (z != 0.0f) unsafeAtomicAdd(dst + k + 2, z);
+        }
+        pending &= ~__ballot(mine);
+    }
+    if ((pending >> __lane_id()) & 1ull) {
+        if (val.x != 0.0f) unsafeAtomicAdd(dst + e, val.x);
+        if (val.y != 0.0f) unsafeAtomicAdd(dst + e + 1, val.y);
+        if (val.z != 0.0f) unsafeAtomicAdd(dst + e + 2, val.z);
+    }
+}
+
+// Gradient of sum_pixels <grad_img, pixel> for one chunk of primary samples.
+// Requires the chunk's primary hits (k_primary) and transmittances
+// (k_shade_primary + k_shadow) of the replayed frame.
+__global__ SHADE_BOUNDS void k_adjoint(DScene S, FrameDesc F, const int4* hit_p, EventBufs EB, const float* grad_img,
+                                       AdjGrads G) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t n_iter = (F.n_slots + stride - 1) / stride;  // whole waves reach every ballot
+    const int L = S.num_lights;
+    const float inv_ns = 1.0f / (float)F.ns;
+    for (uint32_t it = 0; it < n_iter; it++) {
+        const uint32_t i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        int x = 0, y = 0, s = 0;
+        const bool valid = i < F.n_slots && slot_pixel(F, F.g0 + i, x, y, s);
+        float pa = 0.0f;
+        uint32_t seed = 0;
+        DRay r;
+        QHit q;
+        q.kind = -1;
+        v3 g = mk3(0, 0, 0);
+        if (valid) {
+            r = camera_ray(S.cam, x, y, s, F.ns, F.width, F.height, pa, seed);
+            q = unpack_hit(hit_p[i]);
+            const float* gp = grad_img + ((size_t)y * F.width + x) * 4;
+            g = mk3(gp[0], gp[1], gp[2]) * inv_ns;
+        }
+        const bool hit = valid && q.kind >= 0;
+        DHit rec;
+        Material mat;
+        ShadeIn in;
+        float shin = 0.0f;
+        v3 base = mk3(0, 0, 0), dbase = mk3(0, 0, 0);
+        if (hit) {
+            rebuild_hit<false>(S, r, q, rec);
+            shade_setup(S, rec, -r.d, rec.t * pa, mat, in, shin);
+            base = albedo_of(S, mat, in);
+            dbase = mk3(mat.ambient, mat.ambient, mat.ambient);
+        }
+        const uint32_t ev = i * 3u;
+        for (int l = 0; l < L; l++) {
+            float dI = 0.0f;
+            if (hit) {
+                const Light& light = S.lights[l];
+                const LightSample u = light_sample(S, light, 1.0f, seed ^ (uint32_t)(uint32_t(l) * 2654435761u), rec.point);
+                const float tr = u.skip ? 0.0f : EB.trans[(size_t)ev * L + l];
+                if (tr != 0.0f) {
+                    const float diff = fmaxf(dot(in.normal, u.Ld), 0.0f);
+                    float sp = 0.0f;
+                    if (mat.specular > 0.0f && diff > 0.0f)
+                        sp = powf(fmaxf(dot(in.normal, normalize(u.Ld + in.view)), 0.0f), shin);
+                    dbase = dbase + u.rad * (light.intensity * diff * tr);
+                    const v3 dc = (base * u.rad * diff + u.rad * (mat.specular * sp)) * tr;
+                    dI = dot(g, dc);
+                }
+            }
+            dI = wave_sum_f(dI);
+            if (__lane_id() == 0 && dI != 0.0f) unsafeAtomicAdd(G.intensity + l, dI);
+        }
+        const bool textured = hit && tex_ok(S, mat.texture_id);
+        TexTap p;
+        p.e00 = p.e10 = p.e01 = p.e11 = 0;
+        p.tx = p.ty = 0.0f;
+        uint32_t base_e = 0;  // the texture's chain offset: keys are absolute
+        v3 gb = mk3(0, 0, 0);
+        if (textured) {
+            const TextureRef& t = S.textures[mat.texture_id];
+            p = tex_tap(t, in.uvx, in.uvy, tex_lod(t, in.uv_fp));
+            gb = g * dbase;
+            base_e = S.tex_f_off[mat.texture_id];
+        }
+        wave_add_shared(G.tex, textured, base_e + p.e00, gb * ((1.0f - p.tx) * (1.0f - p.ty)));
+        wave_add_shared(G.tex, textured, base_e + p.e10, gb * (p.tx * (1.0f - p.ty)));
+        wave_add_shared(G.tex, textured, base_e + p.e01, gb * ((1.0f - p.tx) * p.ty));
+        wave_add_shared(G.tex, textured, base_e + p.e11, gb * (p.tx * p.ty));
+        const int key = hit ? rec.material_id : -1;
+        wave_add_keyed(G.emission, key, g);
+        wave_add_keyed(G.albedo, textured ? -1 : key, g * dbase);
+    }
+}

@@ -82,6 +82,10 @@ struct DScene {
     int num_materials;
     const TextureRef* textures;
     int num_textures;
+    // float texel chains (same element layout as the 8-bit chains), set by the
+    // differentiable entry points; null for the reference's 8-bit textures
+    const float* tex_f;
+    const uint32_t* tex_f_off;  // per texture: element offset of level 0
     const Light* lights;
     int num_lights;
     const Medium* media;

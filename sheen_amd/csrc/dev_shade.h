@@ -9,19 +9,23 @@
 SHD bool tex_ok(const DScene& S, int id) { return id >= 0 && id < S.num_textures && S.textures[id].pixels; }
 SHD float tex_lod(const TextureRef& t, float uv_fp) { return log2f(fmaxf(uv_fp * (float)t.width, 1.0f)); }
 
-// reference integrator/shading.h:26-58
-SHD v3 tex_sample(const TextureRef& t, float ux, float uy, float lod) {
+// Bilinear tap of a texture's mip chain (reference integrator/shading.h:26-58):
+// nearest level to lod, v flipped, coordinates wrapped.
+struct TexTap {
+    uint32_t e00, e10, e01, e11;  // element offsets (texel * 3) in the chain
+    float tx, ty;
+};
+SHD TexTap tex_tap(const TextureRef& t, float ux, float uy, float lod) {
     int level = (int)floorf(lod + 0.5f);
     if (level < 0) level = 0;
     if (level >= t.mips) level = t.mips - 1;
     int w = t.width, h = t.height;
-    size_t off = 0;
+    uint32_t off = 0;
     for (int i = 0; i < level; i++) {
-        off += (size_t)w * h * 3;
+        off += (uint32_t)w * h * 3u;
         w = (w > 1) ? w / 2 : 1;
         h = (h > 1) ? h / 2 : 1;
     }
-    const unsigned char* base = t.pixels + off;
     float u = ux - floorf(ux);
     float v = 1.0f - (uy - floorf(uy));
     float fx = u * (float)(w - 1);
@@ -29,15 +33,34 @@ SHD v3 tex_sample(const TextureRef& t, float ux, float uy, float lod) {
     int x0 = (int)fx, y0 = (int)fy;
     int x1 = (x0 + 1) % w;
     int y1 = (y0 + 1) % h;
-    float tx = fx - (float)x0, ty = fy - (float)y0;
-    auto at = [&](int x, int y) {
-        const unsigned char* p = base + (size_t)(y * w + x) * 3;
-        return mk3(p[0] / 255.0f, p[1] / 255.0f, p[2] / 255.0f);
-    };
-    v3 c00 = at(x0, y0), c10 = at(x1, y0), c01 = at(x0, y1), c11 = at(x1, y1);
+    TexTap p;
+    p.tx = fx - (float)x0;
+    p.ty = fy - (float)y0;
+    p.e00 = off + (uint32_t)(y0 * w + x0) * 3u;
+    p.e10 = off + (uint32_t)(y0 * w + x1) * 3u;
+    p.e01 = off + (uint32_t)(y1 * w + x0) * 3u;
+    p.e11 = off + (uint32_t)(y1 * w + x1) * 3u;
+    return p;
+}
+SHD v3 tex_blend(v3 c00, v3 c10, v3 c01, v3 c11, float tx, float ty) {
     v3 a = c00 * (1.0f - tx) + c10 * tx;
     v3 b = c01 * (1.0f - tx) + c11 * tx;
     return a * (1.0f - ty) + b * ty;
+}
+// Texture `id` (tex_ok) at uv and lod: 8-bit texels / 255, or the float chain.
+SHD v3 tex_sample(const DScene& S, int id, float ux, float uy, float lod) {
+    const TextureRef& t = S.textures[id];
+    const TexTap p = tex_tap(t, ux, uy, lod);
+    if (S.tex_f) {
+        const float* f = S.tex_f + S.tex_f_off[id];
+        auto at = [&](uint32_t e) { return mk3(f[e], f[e + 1], f[e + 2]); };
+        return tex_blend(at(p.e00), at(p.e10), at(p.e01), at(p.e11), p.tx, p.ty);
+    }
+    auto at = [&](uint32_t e) {
+        const unsigned char* q = t.pixels + e;
+        return mk3(q[0] / 255.0f, q[1] / 255.0f, q[2] / 255.0f);
+    };
+    return tex_blend(at(p.e00), at(p.e10), at(p.e01), at(p.e11), p.tx, p.ty);
 }
 
 struct ShadeIn {
@@ -47,8 +70,7 @@ struct ShadeIn {
 
 SHD v3 albedo_of(const DScene& S, const Material& m, const ShadeIn& in) {
     if (tex_ok(S, m.texture_id)) {
-        const TextureRef& t = S.textures[m.texture_id];
-        return tex_sample(t, in.uvx, in.uvy, tex_lod(t, in.uv_fp));
+        return tex_sample(S, m.texture_id, in.uvx, in.uvy, tex_lod(S.textures[m.texture_id], in.uv_fp));
     }
     return cv3(m.diffuse_color);
 }
@@ -59,11 +81,11 @@ SHD float rough_to_shin(float r) {  // shading.h:85-89
 }
 SHD float transmission_at(const DScene& S, float uvx, float uvy, const Material& m) {  // shading.h:92-99
     if (m.transmission <= 0.0f) return 0.0f;
-    if (tex_ok(S, m.mask_id)) return m.transmission * tex_sample(S.textures[m.mask_id], uvx, uvy, 0.0f).x;
+    if (tex_ok(S, m.mask_id)) return m.transmission * tex_sample(S, m.mask_id, uvx, uvy, 0.0f).x;
     return m.transmission;
 }
 SHD float ior_at(const DScene& S, float uvx, float uvy, const Material& m) {  // shading.h:102-108
-    if (tex_ok(S, m.mask_id)) return 1.0f + tex_sample(S.textures[m.mask_id], uvx, uvy, 0.0f).y;
+    if (tex_ok(S, m.mask_id)) return 1.0f + tex_sample(S, m.mask_id, uvx, uvy, 0.0f).y;
     return m.ior;
 }
 SHD v3 shade_hair(const Material& m, const ShadeIn& in, v3 L, v3 rad) {  // shading.h:112-152
@@ -115,6 +137,84 @@ SHD v3 sun_dir(const Light& light, uint32_t seed) {  // render_kernel.cu:13-24
 SHD v3 sky(v3 d) {  // render_kernel.cu:329-334
     float t = 0.5f * (d.y + 1.0f);
     return mk3((1.0f - t) * 1.0f + t * 0.5f, (1.0f - t) * 1.0f + t * 0.7f, (1.0f - t) * 1.0f + t * 1.0f);
+}
+
+// One light's sample for a shading point (reference render_kernel.cu:150-253):
+// direction, radiance, shadow target and distance bound; skip = no
+// contribution (back of an area light, outside a spot cone).  Radiance is
+// linear in `intensity`, which the caller passes (the light's own, or 1 for
+// the derivative).
+struct LightSample {
+    v3 Ld, rad, target;
+    float t_max;
+    bool skip;
+};
+SHD LightSample light_sample(const DScene& S, const Light& light, float intensity, uint32_t ls, v3 point) {
+    v3 Ld, radiance, target;
+    float t_max = 1e10f;
+    bool skip = false;
+    if (light.type == 0) {
+        Ld = sun_dir(light, ls);
+        radiance = cv3(light.color) * intensity;
+        target = point + Ld * 1.0e6f;
+    } else {
+        v3 sp = cv3(light.position);
+        if (light.type == 2) {
+            float ru = rand01(ls) * 2.0f - 1.0f;
+            float rv = rand01(ls ^ 0x9e3779b9u) * 2.0f - 1.0f;
+            sp = cv3(light.position) + cv3(light.u_axis) * ru + cv3(light.v_axis) * rv;
+        }
+        v3 tl = sp - point;
+        float dist2 = fmaxf(dot(tl, tl), 1.0e-4f);
+        float dist = sqrtf(dist2);
+        Ld = tl / dist;
+        if ((light.type == 1 || light.type == 3) && light.radius > 0.0f) {
+            v3 up = (fabsf(Ld.y) < 0.9f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
+            v3 t1 = normalize(cross(Ld, up));
+            v3 t2 = cross(Ld, t1);
+            float r = sqrtf(rand01(ls ^ 0x85ebca6bu)) * light.radius;
+            float phi = 6.2831853f * rand01(ls ^ 0xc2b2ae35u);
+            sp = sp + t1 * (cosf(phi) * r) + t2 * (sinf(phi) * r);
+            tl = sp - point;
+            dist2 = fmaxf(dot(tl, tl), 1.0e-4f);
+            dist = sqrtf(dist2);
+            Ld = tl / dist;
+        }
+        float atten = intensity / (12.5663706f * dist2);
+        v3 gobo = mk3(1, 1, 1);
+        if (light.type == 2) {
+            v3 n = normalize(cross(cv3(light.u_axis), cv3(light.v_axis)));
+            float cos_l = dot(n, Ld);
+            if (cos_l <= 0.0f) skip = true;
+            else atten *= cos_l;
+        } else if (light.type == 3) {
+            v3 ts = normalize(point - sp);
+            float c = dot(ts, cv3(light.direction));
+            float cos_half = cosf(light.angle);
+            if (c <= cos_half) skip = true;
+            else {
+                float blend = fminf(fmaxf(light.blend, 0.0f), 1.0f);
+                float cos_soft = cosf(light.angle * (1.0f - blend));
+                if (cos_soft > cos_half + 1.0e-4f) atten *= fminf((c - cos_half) / (cos_soft - cos_half), 1.0f);
+                if (tex_ok(S, light.texture_id)) {
+                    v3 axis = cv3(light.direction);
+                    v3 up = (fabsf(axis.y) < 0.9f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
+                    v3 t1 = normalize(cross(axis, up));
+                    v3 t2 = cross(axis, t1);
+                    float tan_half = tanf(fmaxf(light.angle, 1.0e-3f));
+                    float u = 0.5f + 0.5f * dot(ts, t1) / tan_half;
+                    float v = 0.5f + 0.5f * dot(ts, t2) / tan_half;
+                    gobo = tex_sample(S, light.texture_id, u, v, 0.0f);
+                }
+            }
+        }
+        radiance = cv3(light.color) * atten * gobo;
+        target = sp;
+        t_max = dist * 0.999f;
+    }
+    LightSample out;
+    out.Ld = Ld; out.rad = radiance; out.target = target; out.t_max = t_max; out.skip = skip;
+    return out;
 }
 
 // Photon hash grid (reference render_kernel.h:55-111).
@@ -222,6 +322,69 @@ struct EventBufs {
 #define EV_DEBUG 4      // material debug colours; light index in bits 8..
 #define EV_CAUSTIC 8
 
+// Shading inputs of a hit (reference render_kernel.cu:30-146): material,
+// footprint and tangent frame of a triangle hit, normal map, shininess from
+// the mask texture.
+SHD void shade_setup(const DScene& S, const DHit& rec, v3 view_dir, float footprint, Material& mat, ShadeIn& in,
+                     float& shininess) {
+    mat = S.materials[rec.material_id];
+    in.point = rec.point; in.normal = rec.normal; in.tangent = rec.tangent;
+    in.uvx = rec.uvx; in.uvy = rec.uvy; in.view = view_dir;
+    float uv_fp = footprint;
+    v3 tan_u = mk3(0, 0, 0), tan_v = mk3(0, 0, 0);
+    if (rec.object_id < 0 && S.num_triangles > 0) {  // render_kernel.cu:82-114
+        int ti = -rec.object_id - 1;
+        if (ti >= 0 && ti < S.num_triangles) {
+            const Triangle& tr = S.triangles[ti];
+            v3 e1 = cv3(tr.v1) - cv3(tr.v0), e2 = cv3(tr.v2) - cv3(tr.v0);
+            float d1x = tr.uv1.x - tr.uv0.x, d1y = tr.uv1.y - tr.uv0.y;
+            float d2x = tr.uv2.x - tr.uv0.x, d2y = tr.uv2.y - tr.uv0.y;
+            float w1 = sqrtf(dot(e1, e1)), w2 = sqrtf(dot(e2, e2));
+            float u1 = sqrtf(d1x * d1x + d1y * d1y);
+            float u2 = sqrtf(d2x * d2x + d2y * d2y);
+            float a1 = (w1 > 1.0e-6f) ? u1 / w1 : 0.0f;
+            float a2 = (w2 > 1.0e-6f) ? u2 / w2 : 0.0f;
+            float dens = (a1 > a2) ? a1 : a2;
+            if (dens > 1.0e-6f) uv_fp = footprint * dens;
+            float det = d1x * d2y - d2x * d1y;
+            if (fabsf(det) > 1.0e-8f) {
+                v3 du = (e1 * d2y - e2 * d1y) / det;
+                v3 dv = (e2 * d1x - e1 * d2x) / det;
+                v3 n0 = rec.normal;
+                du = du - n0 * dot(du, n0);
+                dv = dv - n0 * dot(dv, n0);
+                float lu = dot(du, du);
+                if (lu > 1.0e-12f) {
+                    tan_u = du * (1.0f / sqrtf(lu));
+                    dv = dv - tan_u * dot(dv, tan_u);
+                    float lv = dot(dv, dv);
+                    if (lv > 1.0e-12f) tan_v = dv * (1.0f / sqrtf(lv));
+                }
+            }
+        }
+    }
+    in.uv_fp = uv_fp;
+    if (tex_ok(S, mat.normal_id)) {  // render_kernel.cu:119-136
+        const TextureRef& nt = S.textures[mat.normal_id];
+        v3 ntex = tex_sample(S, mat.normal_id, in.uvx, in.uvy, tex_lod(nt, in.uv_fp));
+        float nx = (ntex.x * 2.0f - 1.0f) * mat.normal_strength;
+        float ny = (ntex.y * 2.0f - 1.0f) * mat.normal_strength;
+        v3 n0 = in.normal, t1, t2;
+        if (dot(tan_u, tan_u) > 0.5f && dot(tan_v, tan_v) > 0.5f) { t1 = tan_u; t2 = tan_v; }
+        else {
+            v3 up = (fabsf(n0.y) < 0.9f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
+            t1 = normalize(cross(up, n0));
+            t2 = cross(n0, t1);
+        }
+        in.normal = normalize(n0 + t1 * nx + t2 * ny);
+    }
+    shininess = mat.shininess;
+    if (tex_ok(S, mat.mask_id)) {
+        const TextureRef& mt = S.textures[mat.mask_id];
+        shininess = rough_to_shin(tex_sample(S, mat.mask_id, in.uvx, in.uvy, tex_lod(mt, in.uv_fp)).z);
+    }
+}
+
 // compute_lighting (reference render_kernel.cu:26-298) for one event.
 // Writes the unshadowed term, one contribution per light and queues one
 // shadow request per light that reaches the shadow test.  `active` lanes
@@ -234,64 +397,9 @@ SHD void shade_event(const DScene& S, bool active, const DHit& rec, v3 view_dir,
     v3 base = mk3(0, 0, 0), result = mk3(0, 0, 0);
     float shininess = 0.0f;
     if (active) {
-        mat = S.materials[rec.material_id];
-        in.point = rec.point; in.normal = rec.normal; in.tangent = rec.tangent;
-        in.uvx = rec.uvx; in.uvy = rec.uvy; in.view = view_dir;
-        float uv_fp = footprint;
-        v3 tan_u = mk3(0, 0, 0), tan_v = mk3(0, 0, 0);
-        if (rec.object_id < 0 && S.num_triangles > 0) {  // render_kernel.cu:82-114
-            int ti = -rec.object_id - 1;
-            if (ti >= 0 && ti < S.num_triangles) {
-                const Triangle& tr = S.triangles[ti];
-                v3 e1 = cv3(tr.v1) - cv3(tr.v0), e2 = cv3(tr.v2) - cv3(tr.v0);
-                float d1x = tr.uv1.x - tr.uv0.x, d1y = tr.uv1.y - tr.uv0.y;
-                float d2x = tr.uv2.x - tr.uv0.x, d2y = tr.uv2.y - tr.uv0.y;
-                float w1 = sqrtf(dot(e1, e1)), w2 = sqrtf(dot(e2, e2));
-                float u1 = sqrtf(d1x * d1x + d1y * d1y);
-                float u2 = sqrtf(d2x * d2x + d2y * d2y);
-                float a1 = (w1 > 1.0e-6f) ? u1 / w1 : 0.0f;
-                float a2 = (w2 > 1.0e-6f) ? u2 / w2 : 0.0f;
-                float dens = (a1 > a2) ? a1 : a2;
-                if (dens > 1.0e-6f) uv_fp = footprint * dens;
-                float det = d1x * d2y - d2x * d1y;
-                if (fabsf(det) > 1.0e-8f) {
-                    v3 du = (e1 * d2y - e2 * d1y) / det;
-                    v3 dv = (e2 * d1x - e1 * d2x) / det;
-                    v3 n0 = rec.normal;
-                    du = du - n0 * dot(du, n0);
-                    dv = dv - n0 * dot(dv, n0);
-                    float lu = dot(du, du);
-                    if (lu > 1.0e-12f) {
-                        tan_u = du * (1.0f / sqrtf(lu));
-                        dv = dv - tan_u * dot(dv, tan_u);
-                        float lv = dot(dv, dv);
-                        if (lv > 1.0e-12f) tan_v = dv * (1.0f / sqrtf(lv));
-                    }
-                }
-            }
-        }
-        in.uv_fp = uv_fp;
-        if (tex_ok(S, mat.normal_id)) {  // render_kernel.cu:119-136
-            const TextureRef& nt = S.textures[mat.normal_id];
-            v3 ntex = tex_sample(nt, in.uvx, in.uvy, tex_lod(nt, in.uv_fp));
-            float nx = (ntex.x * 2.0f - 1.0f) * mat.normal_strength;
-            float ny = (ntex.y * 2.0f - 1.0f) * mat.normal_strength;
-            v3 n0 = in.normal, t1, t2;
-            if (dot(tan_u, tan_u) > 0.5f && dot(tan_v, tan_v) > 0.5f) { t1 = tan_u; t2 = tan_v; }
-            else {
-                v3 up = (fabsf(n0.y) < 0.9f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
-                t1 = normalize(cross(up, n0));
-                t2 = cross(n0, t1);
-            }
-            in.normal = normalize(n0 + t1 * nx + t2 * ny);
-        }
+        shade_setup(S, rec, view_dir, footprint, mat, in, shininess);
         base = albedo_of(S, mat, in);
         result = base * mat.ambient + cv3(mat.emission_color) * mat.emission;
-        shininess = mat.shininess;
-        if (tex_ok(S, mat.mask_id)) {
-            const TextureRef& mt = S.textures[mat.mask_id];
-            shininess = rough_to_shin(tex_sample(mt, in.uvx, in.uvy, tex_lod(mt, in.uv_fp)).z);
-        }
     }
     const bool debug = active && mat.shininess < 0.0f;
     const bool want_tau = active && mat.anisotropy <= 0.0f && S.count[C_STRAND] > 0;
@@ -303,70 +411,11 @@ SHD void shade_event(const DScene& S, bool active, const DHit& rec, v3 view_dir,
         v3 so = mk3(0, 0, 0), sd = mk3(0, 0, 1);
         float stmax = 0.0f;
         if (active && !(debug && debug_taken)) {
-            const Light& light = S.lights[i];
-            uint32_t ls = seed ^ (uint32_t)(uint32_t(i) * 2654435761u);
-            v3 Ld, radiance, target;
-            float t_max = 1e10f;
-            bool skip = false;
-            if (light.type == 0) {
-                Ld = sun_dir(light, ls);
-                radiance = cv3(light.color) * light.intensity;
-                target = rec.point + Ld * 1.0e6f;
-            } else {
-                v3 sp = cv3(light.position);
-                if (light.type == 2) {
-                    float ru = rand01(ls) * 2.0f - 1.0f;
-                    float rv = rand01(ls ^ 0x9e3779b9u) * 2.0f - 1.0f;
-                    sp = cv3(light.position) + cv3(light.u_axis) * ru + cv3(light.v_axis) * rv;
-                }
-                v3 tl = sp - rec.point;
-                float dist2 = fmaxf(dot(tl, tl), 1.0e-4f);
-                float dist = sqrtf(dist2);
-                Ld = tl / dist;
-                if ((light.type == 1 || light.type == 3) && light.radius > 0.0f) {
-                    v3 up = (fabsf(Ld.y) < 0.9f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
-                    v3 t1 = normalize(cross(Ld, up));
-                    v3 t2 = cross(Ld, t1);
-                    float r = sqrtf(rand01(ls ^ 0x85ebca6bu)) * light.radius;
-                    float phi = 6.2831853f * rand01(ls ^ 0xc2b2ae35u);
-                    sp = sp + t1 * (cosf(phi) * r) + t2 * (sinf(phi) * r);
-                    tl = sp - rec.point;
-                    dist2 = fmaxf(dot(tl, tl), 1.0e-4f);
-                    dist = sqrtf(dist2);
-                    Ld = tl / dist;
-                }
-                float atten = light.intensity / (12.5663706f * dist2);
-                v3 gobo = mk3(1, 1, 1);
-                if (light.type == 2) {
-                    v3 n = normalize(cross(cv3(light.u_axis), cv3(light.v_axis)));
-                    float cos_l = dot(n, Ld);
-                    if (cos_l <= 0.0f) skip = true;
-                    else atten *= cos_l;
-                } else if (light.type == 3) {
-                    v3 ts = normalize(rec.point - sp);
-                    float c = dot(ts, cv3(light.direction));
-                    float cos_half = cosf(light.angle);
-                    if (c <= cos_half) skip = true;
-                    else {
-                        float blend = fminf(fmaxf(light.blend, 0.0f), 1.0f);
-                        float cos_soft = cosf(light.angle * (1.0f - blend));
-                        if (cos_soft > cos_half + 1.0e-4f) atten *= fminf((c - cos_half) / (cos_soft - cos_half), 1.0f);
-                        if (tex_ok(S, light.texture_id)) {
-                            v3 axis = cv3(light.direction);
-                            v3 up = (fabsf(axis.y) < 0.9f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
-                            v3 t1 = normalize(cross(axis, up));
-                            v3 t2 = cross(axis, t1);
-                            float tan_half = tanf(fmaxf(light.angle, 1.0e-3f));
-                            float u = 0.5f + 0.5f * dot(ts, t1) / tan_half;
-                            float v = 0.5f + 0.5f * dot(ts, t2) / tan_half;
-                            gobo = tex_sample(S.textures[light.texture_id], u, v, 0.0f);
-                        }
-                    }
-                }
-                radiance = cv3(light.color) * atten * gobo;
-                target = sp;
-                t_max = dist * 0.999f;
-            }
+            const LightSample ls = light_sample(S, S.lights[i], S.lights[i].intensity, seed ^ (uint32_t)(uint32_t(i) * 2654435761u),
+                                               rec.point);
+            const v3 Ld = ls.Ld, radiance = ls.rad, target = ls.target;
+            const float t_max = ls.t_max;
+            const bool skip = ls.skip;
             if (!skip) {
                 emit = true;
                 so = rec.point + rec.normal * 0.002f;

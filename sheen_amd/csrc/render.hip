@@ -15,6 +15,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include "dev_adjoint.h"
 #include "dev_photon.h"
 #include "kernels.h"
 #include "sheen_c_api.h"
@@ -86,6 +87,18 @@ struct SheenContext {
     // launch shapes: blocks per CU of each grid-stride kernel (0 = one work
     // item per thread); queue-fed kernels are launched over the queue capacity
     int primary_grid = 64, sec_grid = 6, shadow_grid = 64, shade_grid = 2, shade_sec_grid = 2;
+    // differentiable shading: why the scene is not differentiable ("" = it is),
+    // float texel chains, their gradient, and the scene's own materials/lights
+    std::string adj_reason;
+    std::vector<int> tex_w, tex_h, tex_mips;
+    std::vector<uint32_t> tex_off;  // element offset of each texture's level 0 in the chain
+    size_t tex_elems = 0;           // chain elements of all textures
+    float* tex_f = nullptr;
+    uint32_t* tex_f_off = nullptr;
+    float* gtex = nullptr;
+    Material* mats_orig = nullptr;
+    Light* lights_orig = nullptr;
+    hipEvent_t ev_join[2] = {nullptr, nullptr};
     hipEvent_t ev_frame[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -115,8 +128,16 @@ struct SheenContext {
         if (cell_count) (void)hipFree(cell_count);
         for (void* p : owned) (void)hipFree(p);
         for (hipEvent_t e : ev_frame) if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ev_join) if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
+    }
+    template <class T>
+    T* alloc_owned(size_t n) {
+        void* p = nullptr;
+        if (n == 0 || hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+        owned.push_back(p);
+        return (T*)p;
     }
     template <class T>
     T* upload(const T* src, size_t n) {
@@ -231,6 +252,8 @@ static SheenContext* context_create(const SheenScene* in) {
     HIP_OR(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), nullptr);
     HIP_OR(hipEventCreate(&c->ev_frame[0]), nullptr);
     HIP_OR(hipEventCreate(&c->ev_frame[1]), nullptr);
+    HIP_OR(hipEventCreateWithFlags(&c->ev_join[0], hipEventDisableTiming), nullptr);
+    HIP_OR(hipEventCreateWithFlags(&c->ev_join[1], hipEventDisableTiming), nullptr);
 
     HostScene hs;
     std::string err;
@@ -301,6 +324,13 @@ static SheenContext* context_create(const SheenScene* in) {
         }
         d.textures = c->upload(refs.data(), refs.size());
         d.num_textures = in->num_textures;
+        for (int i = 0; i < in->num_textures; i++) {
+            c->tex_w.push_back(refs[i].pixels ? refs[i].width : 0);
+            c->tex_h.push_back(refs[i].pixels ? refs[i].height : 0);
+            c->tex_mips.push_back(refs[i].pixels ? refs[i].mips : 0);
+            c->tex_off.push_back((uint32_t)c->tex_elems);
+            c->tex_elems += hs.tex[i].px.size();
+        }
     }
     d.cam = in->camera;
     d.caustics = in->caustics;
@@ -310,6 +340,19 @@ static SheenContext* context_create(const SheenScene* in) {
     d.any_strands = d.count[C_STRAND] > 0;
     c->full = d.count[C_STRAND] || d.count[C_HF] || d.count[C_SE] || d.count[C_LATHE] || d.count[C_PRISM] ||
               d.csg_root >= 0 || d.num_media > 0 || c->n_beam > 0;
+    // the differentiable path covers primary hits on spheres, triangles and
+    // planes under direct light
+    for (int i = 0; i < in->num_materials && c->adj_reason.empty(); i++) {
+        const Material& m = in->materials[i];
+        if (m.reflect > 0.0f) c->adj_reason = "material with reflection";
+        else if (m.transmission > 0.0f) c->adj_reason = "material with transmission";
+        else if (m.anisotropy > 0.0f) c->adj_reason = "hair material";
+        else if (m.shininess < 0.0f) c->adj_reason = "debug material";
+    }
+    if (c->adj_reason.empty() && d.num_media > 0) c->adj_reason = "participating media";
+    if (c->adj_reason.empty() && c->n_beam > 0) c->adj_reason = "spot-light beams";
+    if (c->adj_reason.empty() && c->full) c->adj_reason = "primitives other than spheres, triangles and planes";
+    if (c->adj_reason.empty() && in->photons > 0 && in->caustics > 0.0f) c->adj_reason = "caustics";
     if (!alloc_dev((void**)&c->ctr, sizeof(Counters))) { sheen_set_error("counter alloc failed"); return nullptr; }
     if (in->photons > 0 && d.num_lights > 0) {
         if (!photon_pass(c.get(), in)) return nullptr;
@@ -377,9 +420,16 @@ struct Timer {
 
 // The kernels of one frame chunk.  FULL = false selects the instantiation
 // compiled for spheres, triangles and planes only.
+// Backward pass of the differentiable entry points: image gradient in, and
+// the gradient buffers the adjoint kernel accumulates into.
+struct AdjCall {
+    const float* grad;
+    AdjGrads G;
+};
+
 template <bool FULL>
 static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, EventBufs& EB, SheenRayStats& S,
-                      bool hits_only, int* dk, int* di, float* dt) {
+                      bool hits_only, int* dk, int* di, float* dt, const AdjCall* adj) {
     WorkBufs& w = c->wb;
     hipStream_t s = c->stream;
     const int ns = F.ns;
@@ -397,6 +447,15 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
         {
             Timer t(c, 4);
             k_shade_primary<FULL><<<grid_for(c, F.n_slots, c->shade_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, SB, EB, w.sec_q[0], c->ctr);
+        }
+        if (adj) {
+            // replay: shadow transmittances of the primary hits, then the adjoint
+            // (differentiable scenes have no secondary rays)
+            const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
+            k_shadow<false, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
+            S.shadow_launches++;
+            k_adjoint<<<grid_for(c, F.n_slots, c->shade_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, EB, adj->grad, adj->G);
+            return;
         }
         // three rounds: reflection + first refraction face, then faces 2 and 3
         for (int round = 0; round < 3; round++) {
@@ -432,7 +491,7 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
 }
 
 static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st, bool hits_only, int* hk, int* hi,
-                        float* ht) {
+                        float* ht, const AdjCall* adj = nullptr) {
     if (W <= 0 || H <= 0 || aa < 1 || aa > 64) { sheen_set_error("bad render params"); return -1; }
     if (c->ds.num_materials <= 0) { sheen_set_error("no materials"); return -1; }
     const int ns = aa;
@@ -489,8 +548,8 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
         F.width = W; F.height = H; F.ns = ns; F.tiles_x = tiles_x;
         F.g0 = (uint32_t)g0;
         F.n_slots = (uint32_t)std::min(chunk, total - g0);
-        if (c->full) run_chunk<true>(c, F, SB, EB, S, hits_only, dk, di, dt);
-        else run_chunk<false>(c, F, SB, EB, S, hits_only, dk, di, dt);
+        if (c->full) run_chunk<true>(c, F, SB, EB, S, hits_only, dk, di, dt, adj);
+        else run_chunk<false>(c, F, SB, EB, S, hits_only, dk, di, dt, adj);
     }
     HIP_OR(hipEventRecord(c->ev_frame[1], s), -1);
     HIP_OR(hipGetLastError(), -1);
@@ -631,6 +690,199 @@ int sheen_context_set_param(SheenContext* ctx, const char* name, int value) {
     else if (n == "chunk") ctx->chunk_override = value > 0 ? value : 0;
     else { sheen_set_error("unknown parameter: %s", name); return -1; }
     return 0;
+}
+
+// ---- differentiable shading ------------------------------------------------
+
+static int adj_check(SheenContext* c) {
+    if (!c->adj_reason.empty()) {
+        sheen_set_error("scene is not differentiable: %s", c->adj_reason.c_str());
+        return -1;
+    }
+    return 0;
+}
+
+// Float chains, gradient chain and the scene's own materials/lights (once).
+static int adj_alloc(SheenContext* c) {
+    if (!c->mats_orig) {
+        c->mats_orig = c->alloc_owned<Material>((size_t)c->ds.num_materials);
+        if (!c->mats_orig) { sheen_set_error("adjoint alloc failed"); return -1; }
+        HIP_OR(hipMemcpyAsync(c->mats_orig, c->ds.materials, sizeof(Material) * c->ds.num_materials, hipMemcpyDeviceToDevice,
+                              c->stream), -1);
+        if (c->ds.num_lights > 0) {
+            c->lights_orig = c->alloc_owned<Light>((size_t)c->ds.num_lights);
+            if (!c->lights_orig) { sheen_set_error("adjoint alloc failed"); return -1; }
+            HIP_OR(hipMemcpyAsync(c->lights_orig, c->ds.lights, sizeof(Light) * c->ds.num_lights, hipMemcpyDeviceToDevice,
+                                  c->stream), -1);
+        }
+    }
+    if (c->tex_elems > 0 && !c->tex_f) {
+        c->tex_f = c->alloc_owned<float>(c->tex_elems);
+        c->gtex = c->alloc_owned<float>(c->tex_elems);
+        c->tex_f_off = c->upload(c->tex_off.data(), c->tex_off.size());
+        if (!c->tex_f || !c->gtex || !c->tex_f_off) { sheen_set_error("adjoint alloc failed"); return -1; }
+    }
+    return 0;
+}
+
+// Makes the render stream wait for the caller's stream (or the reverse).
+static void adj_join(SheenContext* c, hipStream_t user, bool into_render) {
+    if (!user) return;
+    if (into_render) {
+        (void)hipEventRecord(c->ev_join[0], user);
+        (void)hipStreamWaitEvent(c->stream, c->ev_join[0], 0);
+    } else {
+        (void)hipEventRecord(c->ev_join[1], c->stream);
+        (void)hipStreamWaitEvent(user, c->ev_join[1], 0);
+    }
+}
+
+// Parameter set -> device materials, lights and float texel chains.
+static int adj_set(SheenContext* c, const float* albedo, const float* emission, const float* intensity,
+                   const float* texels) {
+    hipStream_t s = c->stream;
+    const int M = c->ds.num_materials, L = c->ds.num_lights;
+    k_adj_set<<<grid_for(c, std::max(M, L), 8), 256, 0, s>>>((Material*)c->ds.materials, M, (Light*)c->ds.lights, L,
+                                                             albedo, emission, intensity);
+    size_t p = 0;
+    for (size_t t = 0; t < c->tex_w.size(); t++) {
+        if (!c->tex_w[t]) continue;
+        int sw = c->tex_w[t], sh = c->tex_h[t];
+        uint32_t src = c->tex_off[t];
+        const size_t n0 = (size_t)sw * sh * 3;
+        HIP_OR(hipMemcpyAsync(c->tex_f + src, texels + p, n0 * sizeof(float), hipMemcpyDeviceToDevice, s), -1);
+        p += n0;
+        for (int lv = 1; lv < c->tex_mips[t]; lv++) {
+            const int nw = sw > 1 ? sw / 2 : 1, nh = sh > 1 ? sh / 2 : 1;
+            const uint32_t dst = src + (uint32_t)sw * sh * 3u;
+            k_mip_f<<<grid_for(c, (size_t)nw * nh, 8), 256, 0, s>>>(c->tex_f, src, sw, sh, dst, nw, nh);
+            src = dst; sw = nw; sh = nh;
+        }
+    }
+    c->ds.tex_f = c->tex_f;
+    c->ds.tex_f_off = c->tex_f_off;
+    return 0;
+}
+
+// Back to the scene's own materials, lights and 8-bit textures.
+static void adj_restore(SheenContext* c) {
+    hipStream_t s = c->stream;
+    (void)hipMemcpyAsync((void*)c->ds.materials, c->mats_orig, sizeof(Material) * c->ds.num_materials,
+                         hipMemcpyDeviceToDevice, s);
+    if (c->lights_orig)
+        (void)hipMemcpyAsync((void*)c->ds.lights, c->lights_orig, sizeof(Light) * c->ds.num_lights, hipMemcpyDeviceToDevice, s);
+    c->ds.tex_f = nullptr;
+    c->ds.tex_f_off = nullptr;
+}
+
+int sheen_adjoint_sizes(SheenContext* ctx, int* materials, int* lights, long long* texels) {
+    if (!ctx) { sheen_set_error("null context"); return -1; }
+    long long n = 0;
+    for (size_t t = 0; t < ctx->tex_w.size(); t++) n += (long long)ctx->tex_w[t] * ctx->tex_h[t] * 3;
+    if (materials) *materials = ctx->ds.num_materials;
+    if (lights) *lights = ctx->ds.num_lights;
+    if (texels) *texels = n;
+    return 0;
+}
+
+int sheen_adjoint_get_params(SheenContext* ctx, float* albedo, float* emission, float* intensity, float* texels,
+                             void* stream) {
+    if (!ctx || !albedo || !emission || (!intensity && ctx->ds.num_lights > 0)) { sheen_set_error("null argument"); return -1; }
+    SheenContext* c = ctx;
+    if (adj_alloc(c) != 0) return -1;
+    adj_join(c, (hipStream_t)stream, true);
+    const int M = c->ds.num_materials, L = c->ds.num_lights;
+    k_adj_get<<<grid_for(c, std::max(M, L), 8), 256, 0, c->stream>>>(c->mats_orig, M, c->lights_orig, L, albedo, emission,
+                                                                      intensity);
+    size_t p = 0;
+    std::vector<TextureRef> refs(c->ds.num_textures);
+    if (c->ds.num_textures > 0)
+        HIP_OR(hipMemcpy(refs.data(), c->ds.textures, sizeof(TextureRef) * refs.size(), hipMemcpyDeviceToHost), -1);
+    for (size_t t = 0; t < c->tex_w.size(); t++) {
+        if (!c->tex_w[t]) continue;
+        const uint32_t n0 = (uint32_t)c->tex_w[t] * c->tex_h[t] * 3u;
+        if (!texels) { sheen_set_error("null texel buffer"); return -1; }
+        k_bytes_to_float<<<grid_for(c, n0, 8), 256, 0, c->stream>>>(refs[t].pixels, n0, texels + p);
+        p += n0;
+    }
+    HIP_OR(hipGetLastError(), -1);
+    adj_join(c, (hipStream_t)stream, false);
+    return 0;
+}
+
+int sheen_adjoint_forward(SheenContext* ctx, int width, int height, int aa_samples, const float* albedo,
+                          const float* emission, const float* intensity, const float* texels, float* image,
+                          void* stream) {
+    if (!ctx || !albedo || !emission || !image) { sheen_set_error("null argument"); return -1; }
+    SheenContext* c = ctx;
+    if (adj_check(c) != 0 || adj_alloc(c) != 0) return -1;
+    adj_join(c, (hipStream_t)stream, true);
+    int rc = adj_set(c, albedo, emission, intensity, texels);
+    if (rc == 0) rc = render_frame(c, width, height, aa_samples, nullptr, false, nullptr, nullptr, nullptr);
+    if (rc == 0 && hipMemcpyAsync(image, c->frame, (size_t)width * height * sizeof(float4), hipMemcpyDeviceToDevice,
+                                  c->stream) != hipSuccess) {
+        sheen_set_error("image copy failed");
+        rc = -1;
+    }
+    adj_restore(c);
+    adj_join(c, (hipStream_t)stream, false);
+    return rc;
+}
+
+int sheen_adjoint_backward(SheenContext* ctx, int width, int height, int aa_samples, const float* albedo,
+                           const float* emission, const float* intensity, const float* texels, const float* grad_image,
+                           float* d_albedo, float* d_emission, float* d_intensity, float* d_texels, void* stream) {
+    if (!ctx || !albedo || !emission || !grad_image || !d_albedo || !d_emission) { sheen_set_error("null argument"); return -1; }
+    SheenContext* c = ctx;
+    if (adj_check(c) != 0 || adj_alloc(c) != 0) return -1;
+    if (c->ds.num_lights > 0 && !d_intensity) { sheen_set_error("null intensity gradient"); return -1; }
+    if (c->tex_elems > 0 && !d_texels) { sheen_set_error("null texel gradient"); return -1; }
+    adj_join(c, (hipStream_t)stream, true);
+    hipStream_t s = c->stream;
+    const int M = c->ds.num_materials, L = c->ds.num_lights;
+    k_fill_f<<<grid_for(c, (size_t)M * 3, 8), 256, 0, s>>>(d_albedo, (size_t)M * 3, 0.0f);
+    k_fill_f<<<grid_for(c, (size_t)M * 3, 8), 256, 0, s>>>(d_emission, (size_t)M * 3, 0.0f);
+    if (L > 0) k_fill_f<<<1, 256, 0, s>>>(d_intensity, (size_t)L, 0.0f);
+    if (c->tex_elems > 0) k_fill_f<<<grid_for(c, c->tex_elems, 8), 256, 0, s>>>(c->gtex, c->tex_elems, 0.0f);
+    int rc = adj_set(c, albedo, emission, intensity, texels);
+    AdjCall call;
+    call.grad = grad_image;
+    call.G.albedo = d_albedo;
+    call.G.emission = d_emission;
+    call.G.intensity = d_intensity;
+    call.G.tex = c->gtex;
+    if (rc == 0) rc = render_frame(c, width, height, aa_samples, nullptr, false, nullptr, nullptr, nullptr, &call);
+    if (rc == 0) {
+        // mip chain adjoint, top level down, then the level-0 gradients out
+        size_t p = 0;
+        for (size_t t = 0; t < c->tex_w.size(); t++) {
+            if (!c->tex_w[t]) continue;
+            std::vector<uint32_t> off;
+            std::vector<int> lw, lh;
+            int sw = c->tex_w[t], sh = c->tex_h[t];
+            uint32_t o = c->tex_off[t];
+            for (int lv = 0; lv < c->tex_mips[t]; lv++) {
+                off.push_back(o); lw.push_back(sw); lh.push_back(sh);
+                o += (uint32_t)sw * sh * 3u;
+                sw = sw > 1 ? sw / 2 : 1;
+                sh = sh > 1 ? sh / 2 : 1;
+            }
+            for (int lv = c->tex_mips[t] - 1; lv >= 1; lv--)
+                k_mip_adj<<<grid_for(c, (size_t)lw[lv] * lh[lv], 8), 256, 0, s>>>(c->gtex, off[lv - 1], lw[lv - 1], lh[lv - 1],
+                                                                                  off[lv], lw[lv], lh[lv]);
+            const size_t n0 = (size_t)c->tex_w[t] * c->tex_h[t] * 3;
+            if (hipMemcpyAsync(d_texels + p, c->gtex + c->tex_off[t], n0 * sizeof(float), hipMemcpyDeviceToDevice, s) != hipSuccess) {
+                sheen_set_error("gradient copy failed");
+                rc = -1;
+                break;
+            }
+            p += n0;
+        }
+    }
+    adj_restore(c);
+    adj_join(c, (hipStream_t)stream, false);
+    if (rc == 0 && hipGetLastError() != hipSuccess) { sheen_set_error("adjoint launch failed"); rc = -1; }
+    return rc;
 }
 
 int sheen_render_float(const SheenScene* scene, int width, int height, int aa_samples, float* out_rgba,

@@ -66,6 +66,10 @@ def capi():
     _sig(lib, "sheen_context_set_counting", C.c_int, [P, C.c_int])
     _sig(lib, "sheen_context_set_chunk", C.c_int, [P, C.c_int])
     _sig(lib, "sheen_context_set_param", C.c_int, [P, C.c_char_p, C.c_int])
+    _sig(lib, "sheen_adjoint_sizes", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)])
+    _sig(lib, "sheen_adjoint_get_params", C.c_int, [P, P, P, P, P, P])
+    _sig(lib, "sheen_adjoint_forward", C.c_int, [P, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P])
+    _sig(lib, "sheen_adjoint_backward", C.c_int, [P, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P])
     _sig(lib, "sheen_context_photon_stats", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)])
     _sig(lib, "sheen_context_photon_map", C.c_int, [P, P, P, P])
     _sig(lib, "sheen_context_photon_probe", C.c_int, [P, C.c_int, C.c_float, C.c_int, C.c_int, P])
