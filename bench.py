@@ -8,6 +8,9 @@ glass, 2 point lights + 1 area light), traced through the full reference path
 is every traversal call of the timed frames divided by their wall time, with
 the scene already resident in HBM.
 
+Also measured (secondary, in the same JSON line): "adjoint", forward and
+backward time of the differentiable-shading workload (adjoint / forward).
+
 Prints one JSON line (see --help).  --gpus above 1 is refused: the path runs
 on one GPU.
 """
@@ -55,12 +58,50 @@ def cpu_baseline(scene, width, height, aa, budget_s=20.0):
     }
 
 
+def adjoint_leg(steps):
+    """Forward and backward (path replay) of the differentiable-shading
+    workload: 512x512, 8 spp, 10,000 spheres with their own albedo, a
+    1024x1024 value-noise texture on the ground plane, 3 lights; gradients of
+    an L2 image loss for albedo, emission, light intensity and texels."""
+    import torch
+    from sheen_amd import adjoint, scenes
+    keys = ("albedo", "emission", "intensity", "texels")
+    sc = scenes.generate(5)
+    r = adjoint.DiffRenderer(sc.scene, sc.width, sc.height, sc.aa)
+    p = {k: v.clone().requires_grad_(True) for k, v in r.params().items()}
+    target = None
+    with torch.no_grad():
+        target = r.render(*(p[k] * 1.1 for k in keys))
+    for _ in range(2):
+        img = r.render(*(p[k] for k in keys))
+        ((img - target) ** 2).sum().backward()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fw = bw = 0.0
+    for _ in range(steps):
+        ev[0].record()
+        img = r.render(*(p[k] for k in keys))
+        ev[1].record()
+        loss = ((img - target) ** 2).sum()
+        torch.autograd.grad(loss, [p[k] for k in keys])
+        ev[2].record()
+        torch.cuda.synchronize()
+        fw += ev[0].elapsed_time(ev[1]) / steps
+        bw += ev[1].elapsed_time(ev[2]) / steps
+    r.close()
+    return {"workload": "diff-shading-512: %d spheres, 1024x1024 texture, %dx%d, %d spp, 3 lights"
+                        % (sc.scene.contents.num_objects, sc.width, sc.height, sc.aa),
+            "forward_ms": round(fw, 3), "backward_ms": round(bw, 3), "adjoint_over_forward": round(bw / fw, 3),
+            "parameters": {k: int(v.numel()) for k, v in p.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-shading leg")
     ap.add_argument("--detail", type=int, default=0, help="geodesic frequency override (default 229)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -149,6 +190,8 @@ def main():
                                                                         "shade_ms")},
     }
     ctx.close()
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_leg(max(3, min(args.steps, 10)))
     if not args.no_cpu_baseline:
         cb = cpu_baseline(sc.scene, W, H, A)
         line["cpu_baseline"] = cb

@@ -108,45 +108,57 @@ SHD float wave_sum_f(float v) {
     return v;
 }
 
-// Adds `val` (3 floats) to dst[3*key] for every lane with key >= 0, one atomic
-// per distinct key per wave.  Every lane of the wave must call it.
-SHD void wave_add_keyed(float* dst, int key, v3 val) {
-    uint64_t pending = __ballot(key >= 0);
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const int k = __shfl(key, leader);
-        const bool mine = key == k;
-        const float x = wave_sum_f(mine ? val.x : 0.0f);
-        const float y = wave_sum_f(mine ? val.y : 0.0f);
-        const float z = wave_sum_f(mine ? val.z : 0.0f);
-        if ((int)__lane_id() == leader) {
-            unsafeAtomicAdd(dst + 3 * k, x);
-            unsafeAtomicAdd(dst + 3 * k + 1, y);
-            unsafeAtomicAdd(dst + 3 * k + 2, z);
-        }
-        pending &= ~__ballot(mine);
+// Per-wave accumulator table: lane j holds entry j (a key and a 3-float sum)
+// in registers.  Wave-reduced sums for a key land in its entry, so a key that
+// every wave meets in every grid-stride iteration (the ground material, a
+// light) takes one atomic per wave at the end instead of one per iteration.
+struct WaveTable {
+    uint32_t key;  // 0xFFFFFFFF = free
+    float x, y, z;
+};
+SHD void table_init(WaveTable& t) { t.key = 0xFFFFFFFFu; t.x = t.y = t.z = 0.0f; }
+
+// Adds a wave-uniform (k, x, y, z) to the table; a full table hands it to
+// dst[k .. k+2] directly.
+SHD void table_add(WaveTable& t, float* dst, uint32_t k, float x, float y, float z) {
+    const int lane = (int)__lane_id();
+    const uint64_t hit = __ballot(t.key == k);
+    int slot = -1;
+    if (hit) slot = __ffsll((unsigned long long)hit) - 1;
+    else {
+        const uint64_t free_ = __ballot(t.key == 0xFFFFFFFFu);
+        if (free_) slot = __ffsll((unsigned long long)free_) - 1;
+    }
+    if (slot >= 0) {
+        if (lane == slot) { t.key = k; t.x += x; t.y += y; t.z += z; }
+    } else if (lane == 0) {
+        if (x != 0.0f) unsafeAtomicAdd(dst + k, x);
+        if (y != 0.0f) unsafeAtomicAdd(dst + k + 1, y);
+        if (z != 0.0f) unsafeAtomicAdd(dst + k + 2, z);
     }
 }
+SHD void table_flush(const WaveTable& t, float* dst) {
+    if (t.key == 0xFFFFFFFFu) return;
+    if (t.x != 0.0f) unsafeAtomicAdd(dst + t.key, t.x);
+    if (t.y != 0.0f) unsafeAtomicAdd(dst + t.key + 1, t.y);
+    if (t.z != 0.0f) unsafeAtomicAdd(dst + t.key + 2, t.z);
+}
 
-// Adds val (3 floats) at dst[e .. e+2] for active lanes.  Lanes of a wave that
-// share an address (a coarse mip level seen by many pixels) are summed first,
-// for up to four distinct addresses, so a hot texel takes one atomic per wave
-// instead of one per lane; the other lanes add directly.  Every lane of the
-// wave must call it.
-SHD void wave_add_shared(float* dst, bool active, uint32_t e, v3 val) {
+// Adds val (3 floats) at dst[e .. e+2] for active lanes.  Lanes sharing an
+// address are summed first (all distinct addresses when `exhaustive`, else
+// the first four, the rest adding directly: addresses met by one lane see
+// little contention) and the sums go through the wave's table.  Every lane
+// of the wave must call it.
+SHD void wave_add(WaveTable& t, float* dst, bool active, uint32_t e, v3 val, bool exhaustive) {
     uint64_t pending = __ballot(active);
-    for (int round = 0; round < 4 && pending; round++) {
+    for (int round = 0; pending && (exhaustive || round < 4); round++) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t k = __shfl(e, leader);
-        const bool mine = active && ((pending >> __lane_id()) & 1ull) && e == k;
+        const bool mine = ((pending >> __lane_id()) & 1ull) && e == k;
         const float x = wave_sum_f(mine ? val.x : 0.0f);
         const float y = wave_sum_f(mine ? val.y : 0.0f);
         const float z = wave_sum_f(mine ? val.z : 0.0f);
-        if ((int)__lane_id() == leader) {
-            if (x != 0.0f) unsafeAtomicAdd(dst + k, x);
-            if (y != 0.0f) unsafeAtomicAdd(dst + k + 1, y);
-            if (z != 0.0f) unsafeAtomicAdd(dst + k + 2, z);
-        }
+        table_add(t, dst, k, x, y, z);
         pending &= ~__ballot(mine);
     }
     if ((pending >> __lane_id()) & 1ull) {
@@ -165,6 +177,8 @@ __global__ SHADE_BOUNDS void k_adjoint(DScene S, FrameDesc F, const int4* hit_p,
     const uint32_t n_iter = (F.n_slots + stride - 1) / stride;  // whole waves reach every ballot
     const int L = S.num_lights;
     const float inv_ns = 1.0f / (float)F.ns;
+    WaveTable t_em, t_alb, t_int, t_tex;
+    table_init(t_em); table_init(t_alb); table_init(t_int); table_init(t_tex);
     for (uint32_t it = 0; it < n_iter; it++) {
         const uint32_t i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
         int x = 0, y = 0, s = 0;
@@ -211,7 +225,7 @@ __global__ SHADE_BOUNDS void k_adjoint(DScene S, FrameDesc F, const int4* hit_p,
                 }
             }
             dI = wave_sum_f(dI);
-            if (__lane_id() == 0 && dI != 0.0f) unsafeAtomicAdd(G.intensity + l, dI);
+            table_add(t_int, G.intensity, (uint32_t)l, dI, 0.0f, 0.0f);
         }
         const bool textured = hit && tex_ok(S, mat.texture_id);
         TexTap p;
@@ -225,12 +239,16 @@ __global__ SHADE_BOUNDS void k_adjoint(DScene S, FrameDesc F, const int4* hit_p,
             gb = g * dbase;
             base_e = S.tex_f_off[mat.texture_id];
         }
-        wave_add_shared(G.tex, textured, base_e + p.e00, gb * ((1.0f - p.tx) * (1.0f - p.ty)));
-        wave_add_shared(G.tex, textured, base_e + p.e10, gb * (p.tx * (1.0f - p.ty)));
-        wave_add_shared(G.tex, textured, base_e + p.e01, gb * ((1.0f - p.tx) * p.ty));
-        wave_add_shared(G.tex, textured, base_e + p.e11, gb * (p.tx * p.ty));
-        const int key = hit ? rec.material_id : -1;
-        wave_add_keyed(G.emission, key, g);
-        wave_add_keyed(G.albedo, textured ? -1 : key, g * dbase);
+        wave_add(t_tex, G.tex, textured, base_e + p.e00, gb * ((1.0f - p.tx) * (1.0f - p.ty)), false);
+        wave_add(t_tex, G.tex, textured, base_e + p.e10, gb * (p.tx * (1.0f - p.ty)), false);
+        wave_add(t_tex, G.tex, textured, base_e + p.e01, gb * ((1.0f - p.tx) * p.ty), false);
+        wave_add(t_tex, G.tex, textured, base_e + p.e11, gb * (p.tx * p.ty), false);
+        const uint32_t mk = hit ? 3u * (uint32_t)rec.material_id : 0u;
+        wave_add(t_em, G.emission, hit, mk, g, false);
+        wave_add(t_alb, G.albedo, hit && !textured, mk, g * dbase, false);
     }
+    table_flush(t_em, G.emission);
+    table_flush(t_alb, G.albedo);
+    table_flush(t_int, G.intensity);
+    table_flush(t_tex, G.tex);
 }

@@ -86,7 +86,7 @@ struct SheenContext {
     long long chunk_override = 0;
     // launch shapes: blocks per CU of each grid-stride kernel (0 = one work
     // item per thread); queue-fed kernels are launched over the queue capacity
-    int primary_grid = 64, sec_grid = 6, shadow_grid = 64, shade_grid = 2, shade_sec_grid = 2;
+    int primary_grid = 64, sec_grid = 6, shadow_grid = 64, shade_grid = 2, shade_sec_grid = 2, adjoint_grid = 8;
     // differentiable shading: why the scene is not differentiable ("" = it is),
     // float texel chains, their gradient, and the scene's own materials/lights
     std::string adj_reason;
@@ -454,7 +454,7 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
             const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
             k_shadow<false, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
             S.shadow_launches++;
-            k_adjoint<<<grid_for(c, F.n_slots, c->shade_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, EB, adj->grad, adj->G);
+            k_adjoint<<<grid_for(c, F.n_slots, c->adjoint_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, EB, adj->grad, adj->G);
             return;
         }
         // three rounds: reflection + first refraction face, then faces 2 and 3
@@ -687,6 +687,7 @@ int sheen_context_set_param(SheenContext* ctx, const char* name, int value) {
     else if (n == "shadow_grid") ctx->shadow_grid = value;
     else if (n == "shade_grid") ctx->shade_grid = value;
     else if (n == "shade_sec_grid") ctx->shade_sec_grid = value;
+    else if (n == "adjoint_grid") ctx->adjoint_grid = value;
     else if (n == "chunk") ctx->chunk_override = value > 0 ? value : 0;
     else { sheen_set_error("unknown parameter: %s", name); return -1; }
     return 0;
