@@ -108,6 +108,11 @@ def main():
         print("bench.py: this render path runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
 
+    if not args.no_adjoint:
+        # torch ships its own HIP runtime; it must initialise before this
+        # library's (ROCm's) when both are used in one process
+        import torch
+        torch.cuda.init()
     from sheen_amd import scenes, sheen
 
     kw = {"detail": args.detail} if args.detail else {}

@@ -185,6 +185,32 @@ int sheen_context_set_chunk(SheenContext* ctx, int samples_per_chunk);
  * value. */
 int sheen_context_set_param(SheenContext* ctx, const char* name, int value);
 
+/* ---- Batches of scenes -------------------------------------------------------
+ * Many scenes (each with its own camera, image size and spp) rendered as one
+ * wavefront pipeline over their concatenated sample slots: per-scene BVHs
+ * and records stay separate (one device table of scene descriptors), queues
+ * and work buffers are shared, and the frame's launches are captured once
+ * into a hipGraph and replayed.  The reference renders one scene per call
+ * (sheen_c_api.cu:187-361); sheen_batch_render_serial renders the same
+ * scenes that way, one render per scene, for comparison.  Scenes must hold
+ * spheres, triangles and planes only, without caustics.
+ */
+typedef struct SheenBatch SheenBatch;
+SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* widths, const int* heights,
+                               const int* aa_samples, int count);
+void sheen_batch_destroy(SheenBatch* batch);
+/* Launch shapes as sheen_context_set_param, plus "graph" (0 = launch the
+ * kernels directly instead of replaying the captured graph). */
+int sheen_batch_set_param(SheenBatch* batch, const char* name, int value);
+/* One frame of every scene.  stats: ray counts of the whole batch, frame_ms
+ * (GPU time, hipEvents) and trace_ms (wall clock of the call). */
+int sheen_batch_render(SheenBatch* batch, SheenRayStats* stats);
+int sheen_batch_render_serial(SheenBatch* batch, SheenRayStats* stats);
+/* Float RGBA image of scene `index` from the last batched (serial = 0) or
+ * serial (serial = 1) render, to host memory. */
+int sheen_batch_image(SheenBatch* batch, int index, int serial, float* out_rgba);
+int sheen_batch_info(SheenBatch* batch, int* scenes, long long* samples, int* chunks);
+
 /* ---- Differentiable shading ------------------------------------------------
  * Forward/backward entries for gradient-based inverse rendering (the
  * reference has none; its ABI lists them as additions a replacement should

@@ -17,6 +17,8 @@
  *      70% diffuse / 15% mirror / 15% glass (IOR 1.5) per triangle,
  *      2 point lights + 1 area light; 1920x1080, 4 spp.
  *   3  analytic primitives + CSG + caustics (see sheen_scenes.cpp).
+ *   4  one scene of the uneven batch: seeded triangle count, resolution
+ *      and spp (sheen_scene_view); the batch uses seeds 0..255 + base.
  *   5  shading: `detail` spheres (default 10,000) with per-sphere albedo
  *      U[0.05,0.95], a 1024x1024 value-noise texture on the ground plane,
  *      3 sun lights with intensity U[0.5,2.0]; 512x512, 8 spp.
@@ -51,6 +53,9 @@ void* sheen_scene_generate(const SheenSceneParams* params);
 
 /* The generated scene; valid until sheen_scene_free. */
 const SheenScene* sheen_scene_get(void* handle);
+/* A generator that picks its own image size and spp (config 4): 1 and the
+ * values, else 0. */
+int sheen_scene_view(void* handle, int* width, int* height, int* aa);
 
 /* Mutable material table of a generated scene (for parameter sweeps). */
 Material* sheen_scene_materials(void* handle, int* count);

@@ -12,7 +12,9 @@ fixed).  Parameters (float32 CUDA tensors):
     texels     (texels,)        level-0 texels of every texture, RGB / 255
 
 torch supplies device memory and the stream; the arithmetic is in
-`sheen_amd/csrc/dev_adjoint.h`.
+`sheen_amd/csrc/dev_adjoint.h`.  torch ships its own HIP runtime next to the
+system ROCm one this library links: import this module (or torch, and call
+torch.cuda.init()) before any sheen context is created in the process.
 """
 import ctypes as C
 

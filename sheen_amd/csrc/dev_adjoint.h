@@ -213,7 +213,7 @@ __global__ SHADE_BOUNDS void k_adjoint(DScene S, FrameDesc F, const int4* hit_p,
             if (hit) {
                 const Light& light = S.lights[l];
                 const LightSample u = light_sample(S, light, 1.0f, seed ^ (uint32_t)(uint32_t(l) * 2654435761u), rec.point);
-                const float tr = u.skip ? 0.0f : EB.trans[(size_t)ev * L + l];
+                const float tr = u.skip ? 0.0f : EB.trans[(size_t)ev * EB.L + l];
                 if (tr != 0.0f) {
                     const float diff = fmaxf(dot(in.normal, u.Ld), 0.0f);
                     float sp = 0.0f;

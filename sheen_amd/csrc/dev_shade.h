@@ -314,6 +314,7 @@ struct EventBufs {
     float* trans;     // [events * L]: transmittance, written by the shadow kernel
     ShadowReq* shadow_q;
     uint32_t* shadow_n;
+    int L;  // lights per event in contrib/trans (a batch: the most of any scene)
 };
 // event kind bits (stored in base.w as int bits)
 #define EV_NONE 0
@@ -391,7 +392,7 @@ SHD void shade_setup(const DScene& S, const DHit& rec, v3 view_dir, float footpr
 // only; every lane of the wave must call this (it reserves queue slots).
 SHD void shade_event(const DScene& S, bool active, const DHit& rec, v3 view_dir, float footprint, uint32_t seed,
                      uint32_t ev, const EventBufs& B) {
-    const int L = S.num_lights;
+    const int L = B.L;  // block-uniform trip count: the loop reserves queue slots
     Material mat;
     ShadeIn in;
     v3 base = mk3(0, 0, 0), result = mk3(0, 0, 0);
@@ -410,7 +411,8 @@ SHD void shade_event(const DScene& S, bool active, const DHit& rec, v3 view_dir,
         v3 contrib = mk3(0, 0, 0);
         v3 so = mk3(0, 0, 0), sd = mk3(0, 0, 1);
         float stmax = 0.0f;
-        if (active && !(debug && debug_taken)) {
+        const bool has_light = active && i < S.num_lights;
+        if (has_light && !(debug && debug_taken)) {
             const LightSample ls = light_sample(S, S.lights[i], S.lights[i].intensity, seed ^ (uint32_t)(uint32_t(i) * 2654435761u),
                                                rec.point);
             const v3 Ld = ls.Ld, radiance = ls.rad, target = ls.target;
@@ -425,7 +427,7 @@ SHD void shade_event(const DScene& S, bool active, const DHit& rec, v3 view_dir,
                 else contrib = shade_direct(S, mat, in, Ld, radiance, shininess);
             }
         }
-        if (active) {
+        if (has_light) {
             B.contrib[(size_t)ev * L + i] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
             B.trans[(size_t)ev * L + i] = 0.0f;
         }
@@ -466,12 +468,12 @@ SHD v3 event_value(const DScene& S, uint32_t ev, const EventBufs& B) {
     int kind = __float_as_int(b.w);
     v3 r = mk3(b.x, b.y, b.z);
     if (!(kind & EV_LIT)) return r;
-    const int L = S.num_lights;
+    const int L = B.L;
     if (kind & EV_DEBUG) {
         int li = kind >> 8;
         return (B.trans[(size_t)ev * L + li] > 0.0f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
     }
-    for (int i = 0; i < L; i++) {
+    for (int i = 0; i < S.num_lights; i++) {
         float tr = B.trans[(size_t)ev * L + i];
         if (tr != 0.0f) {
             float4 c = B.contrib[(size_t)ev * L + i];

@@ -13,8 +13,9 @@
 struct FrameDesc {
     int width, height, ns;
     int tiles_x;
-    uint32_t g0;       // first slot of the chunk
-    uint32_t n_slots;  // slots in the chunk (multiple of 64*ns)
+    uint32_t g0;       // one scene: first slot of the chunk; a batch: the scene's first slot
+    uint32_t n_slots;  // one scene: slots in the chunk (multiple of 64*ns); a batch: the scene's slots
+    uint32_t out0;     // first pixel of the image in the output buffer
 };
 
 // Sample slot -> pixel / sample index.
@@ -304,6 +305,61 @@ SHD void refract_from(const DScene& S, const DHit& gh, v3 d, bool entering, floa
     o_out = gh.point + td * 0.002f;
 }
 
+// Where a chunk's samples come from.  OneSrc: one scene, passed as the
+// kernel argument, chunk slots [F.g0, F.g0 + F.n_slots) of its slot space.
+// ManySrc: a batch; the scenes' slot ranges are concatenated (each a
+// multiple of 64, so a wave's 64 slots belong to one scene), the chunk is
+// [g0, g0 + n) of that space, and a slot's scene is found by binary search
+// over the scenes' first slots.  Kernels use:
+//   n_slots()       slots of the chunk;
+//   find(i)         scene of chunk slot i (i < n_slots());
+//   scene(k), frame(k), local(k, i)  the scene, its frame, i in its slot space;
+//   pix_count(), pix_first(p, i0)    k_accumulate's enumeration of pixels.
+struct OneSrc {
+    DScene S;
+    FrameDesc F;
+    SHD uint32_t n_slots() const { return F.n_slots; }
+    SHD int find(uint32_t) const { return 0; }
+    SHD int find_wave(uint32_t) const { return 0; }
+    SHD const DScene& scene(int) const { return S; }
+    SHD const FrameDesc& frame(int) const { return F; }
+    SHD uint32_t local(int, uint32_t i) const { return F.g0 + i; }
+    SHD uint32_t pix_count() const { return F.n_slots / (uint32_t)F.ns; }
+    // pixel p of the chunk: lane p&63 of tile p>>6; its samples at i0 + s*64
+    SHD bool pix_first(uint32_t p, uint32_t& i0) const {
+        i0 = (p >> 6) * 64u * (uint32_t)F.ns + (p & 63u);
+        return true;
+    }
+};
+struct ManySrc {
+    const DScene* S;
+    const FrameDesc* F;
+    const uint32_t* base;  // [n + 1] first slot of each scene, then the total
+    int n;
+    uint32_t g0, count;    // the chunk
+    SHD uint32_t n_slots() const { return count; }
+    SHD int find(uint32_t i) const {
+        const uint32_t g = g0 + i;
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (base[mid] <= g) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    }
+    // every lane of the wave holds a slot of the same scene
+    SHD int find_wave(uint32_t i) const { return __builtin_amdgcn_readfirstlane(find(i)); }
+    SHD const DScene& scene(int k) const { return S[k]; }
+    SHD const FrameDesc& frame(int k) const { return F[k]; }
+    SHD uint32_t local(int k, uint32_t i) const { return g0 + i - base[k]; }
+    SHD uint32_t pix_count() const { return count; }
+    SHD bool pix_first(uint32_t p, uint32_t& i0) const {
+        i0 = p;
+        const int k = find(p);
+        return ((local(k, p) >> 6) % (uint32_t)F[k].ns) == 0u;
+    }
+};
+
 // Every kernel that traverses declares its LDS columns and its private
 // spill array, then builds the stack view over them.
 #define SHEEN_STACK(st)                                   \
@@ -320,15 +376,19 @@ SHD int4 pack_hit(const QHit& q) {
     return make_int4(__float_as_int(q.t), q.idx, q.carry, q.kind);
 }
 
-template <bool COUNT, bool FULL>
-__global__ TRACE_BOUNDS void k_primary(DScene S, FrameDesc F, int4* hit_p, Counters* C) {
+template <bool COUNT, bool FULL, class SRC>
+__global__ TRACE_BOUNDS void k_primary(SRC src, int4* hit_p, Counters* C) {
     SHEEN_STACK(st);
     TraceStats ts{0, 0};
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < F.n_slots; i += stride) {
+    const uint32_t n = src.n_slots();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int sid = src.find_wave(i);
+        const DScene& S = src.scene(sid);
+        const FrameDesc& F = src.frame(sid);
         int x, y, s;
         int4 out = make_int4(0, -1, 0, -1);
-        if (slot_pixel(F, F.g0 + i, x, y, s)) {
+        if (slot_pixel(F, src.local(sid, i), x, y, s)) {
             float pa;
             uint32_t seed;
             DRay r = camera_ray(S.cam, x, y, s, F.ns, F.width, F.height, pa, seed);
@@ -356,18 +416,21 @@ SHD QHit unpack_hit(int4 h) {
     return q;
 }
 
-template <bool FULL>
-__global__ SHADE_BOUNDS void k_shade_primary(DScene S, FrameDesc F, SampleBufs SB, EventBufs EB,
-                                                               SecRay* sec_q, Counters* C) {
+template <bool FULL, class SRC>
+__global__ SHADE_BOUNDS void k_shade_primary(SRC src, SampleBufs SB, EventBufs EB, SecRay* sec_q, Counters* C) {
     SHEEN_STACK(st);
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t passes = 0;
+    const uint32_t n = src.n_slots();
     // loop bound rounded up to whole waves so every lane reaches the queue ballots
-    const uint32_t n_iter = (F.n_slots + stride - 1) / stride;
+    const uint32_t n_iter = (n + stride - 1) / stride;
     for (uint32_t it = 0; it < n_iter; it++) {
         const uint32_t i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        const int sid = src.find_wave(i < n ? i : n - 1);
+        const DScene& S = src.scene(sid);
+        const FrameDesc& F = src.frame(sid);
         int x = 0, y = 0, s = 0;
-        const bool valid = i < F.n_slots && slot_pixel(F, F.g0 + i, x, y, s);
+        const bool valid = i < n && slot_pixel(F, src.local(sid, i), x, y, s);
         float pa = 0.0f;
         uint32_t seed = 0;
         DRay r;
@@ -419,7 +482,7 @@ __global__ SHADE_BOUNDS void k_shade_primary(DScene S, FrameDesc F, SampleBufs S
         const uint32_t a = block_reserve(&C->sec_n[0], want_r, want_t);
         if (want_r) sec_q[a] = rr;
         if (want_t) sec_q[a + (want_r ? 1u : 0u)] = tr;
-        if (i < F.n_slots) {
+        if (i < n) {
             sp.w = __int_as_float(flags);
             SB.samp[i] = sp;
         }
@@ -427,9 +490,8 @@ __global__ SHADE_BOUNDS void k_shade_primary(DScene S, FrameDesc F, SampleBufs S
     wave_add64(&C->extra_shadow, passes);
 }
 
-template <bool COUNT, bool FULL>
-__global__ TRACE_BOUNDS void k_trace_sec(DScene S, const SecRay* q_in, int which, int4* hits,
-                                                          Counters* C) {
+template <bool COUNT, bool FULL, class SRC>
+__global__ TRACE_BOUNDS void k_trace_sec(SRC src, const SecRay* q_in, int which, int4* hits, Counters* C) {
     SHEEN_STACK(st);
     TraceStats ts{0, 0};
     const uint32_t n = C->sec_n[which];
@@ -437,6 +499,7 @@ __global__ TRACE_BOUNDS void k_trace_sec(DScene S, const SecRay* q_in, int which
     uint32_t nrefl = 0, nrefr = 0;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         SecRay e = q_in[j];
+        const DScene& S = src.scene(src.find(e.meta & 0x0FFFFFFFu));
         DRay r;
         r.o = mk3(e.ox, e.oy, e.oz);
         r.d = mk3(e.dx, e.dy, e.dz);
@@ -455,21 +518,14 @@ __global__ TRACE_BOUNDS void k_trace_sec(DScene S, const SecRay* q_in, int which
     }
 }
 
-template <bool FULL>
-__global__ SHADE_BOUNDS void k_shade_sec(DScene S, FrameDesc F, SampleBufs SB, EventBufs EB,
-                                                          const SecRay* q_in, int which, const int4* hits,
-                                                          SecRay* q_out, int which_out, Counters* C) {
+template <bool FULL, class SRC>
+__global__ SHADE_BOUNDS void k_shade_sec(SRC src, SampleBufs SB, EventBufs EB, const SecRay* q_in, int which,
+                                         const int4* hits, SecRay* q_out, int which_out, Counters* C) {
     SHEEN_STACK(st);
     const uint32_t n = C->sec_n[which];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t n_iter = (n + stride - 1) / stride;
     uint32_t passes = 0;
-    float tan_x = 1.0f;
-    if (S.cam.focal_length > 0.0f) {
-        float tan_h = 0.5f * S.cam.sensor_width / S.cam.focal_length;
-        tan_x = (F.width >= F.height) ? tan_h : tan_h * (float)F.width / (float)F.height;
-    }
-    const float pa = 2.0f * tan_x / (float)F.width;
     for (uint32_t it = 0; it < n_iter; it++) {
         const uint32_t j = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
         const bool valid = j < n;
@@ -489,10 +545,19 @@ __global__ SHADE_BOUNDS void k_shade_sec(DScene S, FrameDesc F, SampleBufs SB, E
             r.tmin = 0.001f;
             r.tmax = 1.0e10f;
         }
+        const int sid = src.find(i);
+        const DScene& S = src.scene(sid);
+        const FrameDesc& F = src.frame(sid);
+        float tan_x = 1.0f;  // pixel footprint angle of the scene's camera
+        if (S.cam.focal_length > 0.0f) {
+            float tan_h = 0.5f * S.cam.sensor_width / S.cam.focal_length;
+            tan_x = (F.width >= F.height) ? tan_h : tan_h * (float)F.width / (float)F.height;
+        }
+        const float pa = 2.0f * tan_x / (float)F.width;
         int x = 0, y = 0, s = 0;
         uint32_t seed = 0;
         if (valid) {
-            slot_pixel(F, F.g0 + i, x, y, s);
+            slot_pixel(F, src.local(sid, i), x, y, s);
             seed = mix32((uint32_t(x) * 73856093u) ^ (uint32_t(y) * 19349663u) ^ (uint32_t(s) * 83492791u));
         }
         const bool hit = valid && q.kind >= 0;
@@ -542,8 +607,8 @@ __global__ SHADE_BOUNDS void k_shade_sec(DScene S, FrameDesc F, SampleBufs SB, E
     wave_add64(&C->extra_shadow, passes);
 }
 
-template <bool COUNT, bool FULL>
-__global__ TRACE_BOUNDS void k_shadow(DScene S, EventBufs EB, Counters* C) {
+template <bool COUNT, bool FULL, class SRC>
+__global__ TRACE_BOUNDS void k_shadow(SRC src, EventBufs EB, Counters* C) {
     SHEEN_STACK(st);
     TraceStats ts{0, 0};
     const uint32_t n = C->shadow_n;
@@ -551,6 +616,8 @@ __global__ TRACE_BOUNDS void k_shadow(DScene S, EventBufs EB, Counters* C) {
     uint32_t passes = 0;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         ShadowReq q = EB.shadow_q[j];
+        // request slot = event * L + light, event = sample slot * 3 + kind
+        const DScene& S = src.scene(src.find((q.slot & 0x7FFFFFFFu) / (uint32_t)EB.L / 3u));
         float tr = shadow_eval<COUNT, FULL>(S, q, st, passes, ts);
         EB.trans[q.slot & 0x7FFFFFFFu] = tr;
     }
@@ -563,15 +630,18 @@ __global__ TRACE_BOUNDS void k_shadow(DScene S, EventBufs EB, Counters* C) {
 }
 
 // Per-pixel accumulation (reference render_kernel.cu:536-813 blend order).
-__global__ __launch_bounds__(256) void k_accumulate(DScene S, FrameDesc F, SampleBufs SB, EventBufs EB,
-                                                   float4* frame) {
-    const uint32_t n_pix = F.n_slots / (uint32_t)F.ns;  // tiles*64 pixels of the chunk
+template <class SRC>
+__global__ __launch_bounds__(256) void k_accumulate(SRC src, SampleBufs SB, EventBufs EB, float4* frame) {
+    const uint32_t n_pix = src.pix_count();
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pix; p += stride) {
-        const uint32_t lane = p & 63u, tile_local = p >> 6;
+        uint32_t i0;
+        if (!src.pix_first(p, i0)) continue;
+        const int sid = src.find(i0);
+        const DScene& S = src.scene(sid);
+        const FrameDesc& F = src.frame(sid);
         int x = 0, y = 0, s0 = 0;
-        const uint32_t i0 = tile_local * 64u * (uint32_t)F.ns + lane;
-        if (!slot_pixel(F, F.g0 + i0, x, y, s0)) continue;
+        if (!slot_pixel(F, src.local(sid, i0), x, y, s0)) continue;
         v3 acc = mk3(0, 0, 0);
         for (int s = 0; s < F.ns; s++) {
             const uint32_t i = i0 + (uint32_t)s * 64u;
@@ -603,7 +673,7 @@ __global__ __launch_bounds__(256) void k_accumulate(DScene S, FrameDesc F, Sampl
             acc = acc + c;
         }
         v3 f = acc * (1.0f / (float)F.ns);
-        frame[(size_t)y * F.width + x] = make_float4(f.x, f.y, f.z, 1.0f);
+        frame[(size_t)F.out0 + (size_t)y * F.width + x] = make_float4(f.x, f.y, f.z, 1.0f);
     }
 }
 
@@ -612,6 +682,12 @@ __global__ void k_reset(Counters* C) {
         for (int k = 0; k < 4; k++) C->sec_n[k] = 0;
         C->shadow_n = 0;
     }
+}
+// Every counter of a frame to zero (a kernel, so that captured graphs hold no
+// memset nodes).
+__global__ void k_zero_counters(Counters* C) {
+    unsigned int* w = reinterpret_cast<unsigned int*>(C);
+    for (unsigned int i = threadIdx.x; i < sizeof(Counters) / 4u; i += blockDim.x) w[i] = 0u;
 }
 __global__ void k_clear_sec(Counters* C, int which) {
     if (threadIdx.x == 0 && blockIdx.x == 0) C->sec_n[which] = 0;

@@ -78,6 +78,7 @@ struct SheenContext {
     int n_beam = 0;
     bool full = true;  // scene uses a kind beyond spheres, triangles and planes
     WorkBufs wb;
+    WorkBufs* wbp = &wb;  // the work buffers in use (a batch shares one set among its scenes)
     float4* frame = nullptr;
     size_t frame_cap = 0;
     int frame_w = 0, frame_h = 0;
@@ -150,14 +151,15 @@ struct SheenContext {
     }
 };
 
-static int grid_for(SheenContext* c, size_t n, int per_cu) {
+static int grid_n(int num_cu, size_t n, int per_cu) {
     size_t blocks = (n + TRACE_BLOCK - 1) / TRACE_BLOCK;
-    size_t cap = per_cu > 0 ? (size_t)c->num_cu * per_cu : blocks;
+    size_t cap = per_cu > 0 ? (size_t)num_cu * per_cu : blocks;
     if (blocks > cap) blocks = cap;
     if (blocks > 0x7FFFFFFF) blocks = 0x7FFFFFFF;
     if (blocks < 1) blocks = 1;
     return (int)blocks;
 }
+static int grid_for(SheenContext* c, size_t n, int per_cu) { return grid_n(c->num_cu, n, per_cu); }
 
 static bool alloc_dev(void** p, size_t bytes) {
     *p = nullptr;
@@ -369,13 +371,11 @@ static size_t slot_bytes(int L, int n_beam, bool caustic, bool media) {
     return b;
 }
 
-static bool ensure_work(SheenContext* c, size_t slots) {
-    WorkBufs& w = c->wb;
-    const int L = c->ds.num_lights;
-    const bool caustic = c->ds.pm.photons && c->ds.caustics > 0.0f;
-    const bool media = c->ds.num_media > 0;
-    if (w.cap_slots >= slots && w.cap_L == L && w.cap_beam == c->n_beam && w.cap_caustic == caustic &&
-        w.cap_media == media)
+// Work buffers for `slots` sample slots of scenes with at most L lights and
+// n_beam beams; kept when the current ones are large enough.
+static bool ensure_bufs(WorkBufs& w, size_t slots, int L, int n_beam, bool caustic, bool media) {
+    if (w.cap_slots >= slots && w.cap_L >= L && w.cap_beam >= n_beam && (w.cap_caustic || !caustic) &&
+        (w.cap_media || !media))
         return true;
     w.release();
     const size_t ev = slots * 3;
@@ -389,15 +389,20 @@ static bool ensure_work(SheenContext* c, size_t slots) {
               alloc_dev((void**)&w.sec_q[1], 2 * slots * sizeof(SecRay)) &&
               alloc_dev((void**)&w.sec_hits, 2 * slots * sizeof(int4));
     if (ok && caustic) ok = alloc_dev((void**)&w.ev_caustic, ev * sizeof(float4));
-    if (ok && c->n_beam > 0) ok = alloc_dev((void**)&w.beam_add, slots * c->n_beam * sizeof(float4));
+    if (ok && n_beam > 0) ok = alloc_dev((void**)&w.beam_add, slots * n_beam * sizeof(float4));
     if (ok && media) ok = alloc_dev((void**)&w.media_p, slots * sizeof(float4)) && alloc_dev((void**)&w.media_r, slots * sizeof(float4));
     if (!ok) {
         w.release();
         sheen_set_error("work buffer allocation failed (%zu slots)", slots);
         return false;
     }
-    w.cap_slots = slots; w.cap_L = L; w.cap_beam = c->n_beam; w.cap_caustic = caustic; w.cap_media = media;
+    w.cap_slots = slots; w.cap_L = L; w.cap_beam = n_beam; w.cap_caustic = caustic; w.cap_media = media;
     return true;
+}
+
+static bool ensure_work(SheenContext* c, size_t slots) {
+    return ensure_bufs(*c->wbp, slots, c->ds.num_lights, c->n_beam, c->ds.pm.photons && c->ds.caustics > 0.0f,
+                       c->ds.num_media > 0);
 }
 
 // Kernel timing with events recorded on the render stream; elapsed times are
@@ -430,14 +435,17 @@ struct AdjCall {
 template <bool FULL>
 static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, EventBufs& EB, SheenRayStats& S,
                       bool hits_only, int* dk, int* di, float* dt, const AdjCall* adj) {
-    WorkBufs& w = c->wb;
+    WorkBufs& w = *c->wbp;
     hipStream_t s = c->stream;
     const int ns = F.ns;
+    OneSrc src;
+    src.S = c->ds;
+    src.F = F;
         k_reset<<<1, 64, 0, s>>>(c->ctr);
         {
             Timer t(c, 0);
-            if (c->counting) k_primary<true, FULL><<<grid_for(c, F.n_slots, c->primary_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, c->ctr);
-            else k_primary<false, FULL><<<grid_for(c, F.n_slots, c->primary_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, c->ctr);
+            if (c->counting) k_primary<true, FULL><<<grid_for(c, F.n_slots, c->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, c->ctr);
+            else k_primary<false, FULL><<<grid_for(c, F.n_slots, c->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, c->ctr);
             S.trace_launches++;
         }
         if (hits_only) {
@@ -446,13 +454,13 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
         }
         {
             Timer t(c, 4);
-            k_shade_primary<FULL><<<grid_for(c, F.n_slots, c->shade_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, SB, EB, w.sec_q[0], c->ctr);
+            k_shade_primary<FULL><<<grid_for(c, F.n_slots, c->shade_grid), TRACE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[0], c->ctr);
         }
         if (adj) {
             // replay: shadow transmittances of the primary hits, then the adjoint
             // (differentiable scenes have no secondary rays)
             const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
-            k_shadow<false, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
+            k_shadow<false, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(src, EB, c->ctr);
             S.shadow_launches++;
             k_adjoint<<<grid_for(c, F.n_slots, c->adjoint_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, EB, adj->grad, adj->G);
             return;
@@ -465,14 +473,14 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
             {
                 Timer t(c, 1);
                 if (c->counting)
-                    k_trace_sec<true, FULL><<<grid_for(c, cap, c->sec_grid), TRACE_BLOCK, 0, s>>>(c->ds, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
+                    k_trace_sec<true, FULL><<<grid_for(c, cap, c->sec_grid), TRACE_BLOCK, 0, s>>>(src, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
                 else
-                    k_trace_sec<false, FULL><<<grid_for(c, cap, c->sec_grid), TRACE_BLOCK, 0, s>>>(c->ds, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
+                    k_trace_sec<false, FULL><<<grid_for(c, cap, c->sec_grid), TRACE_BLOCK, 0, s>>>(src, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
                 S.trace_launches++;
             }
             {
                 Timer t(c, 3);
-                k_shade_sec<FULL><<<grid_for(c, cap, c->shade_sec_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
+                k_shade_sec<FULL><<<grid_for(c, cap, c->shade_sec_grid), TRACE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
                                                                        w.sec_q[out_q], out_ctr, c->ctr);
             }
         }
@@ -480,13 +488,13 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
             Timer t(c, 2);
             const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
             const int g = grid_for(c, cap, c->shadow_grid);
-            if (c->counting) k_shadow<true, FULL><<<g, TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
-            else k_shadow<false, FULL><<<g, TRACE_BLOCK, 0, s>>>(c->ds, EB, c->ctr);
+            if (c->counting) k_shadow<true, FULL><<<g, TRACE_BLOCK, 0, s>>>(src, EB, c->ctr);
+            else k_shadow<false, FULL><<<g, TRACE_BLOCK, 0, s>>>(src, EB, c->ctr);
             S.shadow_launches++;
         }
         {
             Timer t(c, 3);
-            k_accumulate<<<grid_for(c, F.n_slots / ns, 8), 256, 0, s>>>(c->ds, F, SB, EB, c->frame);
+            k_accumulate<<<grid_for(c, F.n_slots / ns, 8), 256, 0, s>>>(src, SB, EB, c->frame);
         }
 }
 
@@ -529,10 +537,11 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
         }
     }
     SheenRayStats S{};
-    WorkBufs& w = c->wb;
+    WorkBufs& w = *c->wbp;
     EventBufs EB;
     EB.base = w.ev_base; EB.caustic = caustic ? w.ev_caustic : nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
     EB.shadow_q = w.shadow_q; EB.shadow_n = &c->ctr->shadow_n;
+    EB.L = c->ds.num_lights;
     SampleBufs SB;
     SB.hit_p = w.hit_p; SB.samp = w.samp; SB.beam_add = c->n_beam ? w.beam_add : nullptr;
     SB.media_p = c->ds.num_media ? w.media_p : nullptr; SB.media_r = c->ds.num_media ? w.media_r : nullptr;
@@ -548,6 +557,7 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
         F.width = W; F.height = H; F.ns = ns; F.tiles_x = tiles_x;
         F.g0 = (uint32_t)g0;
         F.n_slots = (uint32_t)std::min(chunk, total - g0);
+        F.out0 = 0;
         if (c->full) run_chunk<true>(c, F, SB, EB, S, hits_only, dk, di, dt, adj);
         else run_chunk<false>(c, F, SB, EB, S, hits_only, dk, di, dt, adj);
     }
@@ -884,6 +894,273 @@ int sheen_adjoint_backward(SheenContext* ctx, int width, int height, int aa_samp
     adj_join(c, (hipStream_t)stream, false);
     if (rc == 0 && hipGetLastError() != hipSuccess) { sheen_set_error("adjoint launch failed"); rc = -1; }
     return rc;
+}
+
+// ---- batches of scenes ----------------------------------------------------
+// Every scene keeps its own context (BVHs, records, camera); the batch adds a
+// device table of their DScene records and frames and renders all of them as
+// one wavefront pipeline over the concatenated sample slots, captured once
+// into a hipGraph and replayed.  The scenes' own contexts still render one
+// by one (the serial path), sharing one set of work buffers.
+
+}  // extern "C"
+
+struct SheenBatch {
+    int n = 0;
+    int num_cu = 256;
+    std::vector<SheenContext*> ctx;
+    std::vector<FrameDesc> frames;
+    std::vector<uint32_t> base;  // n + 1 first slots
+    std::vector<std::pair<uint32_t, uint32_t>> chunks;
+    uint64_t samples = 0;
+    size_t pixels = 0;
+    int L = 1;
+    DScene* d_scenes = nullptr;
+    FrameDesc* d_frames = nullptr;
+    uint32_t* d_base = nullptr;
+    float4* frame = nullptr;
+    Counters* ctr = nullptr;
+    WorkBufs wb, serial_wb;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int primary_grid = 64, sec_grid = 6, shadow_grid = 64, shade_grid = 2, shade_sec_grid = 2;
+    size_t chunk_slots = 0;
+    bool use_graph = true, dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~SheenBatch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (SheenContext* c : ctx) delete c;
+        wb.release();
+        serial_wb.release();
+        void* ps[] = {d_scenes, d_frames, d_base, frame, ctr};
+        for (void* p : ps) if (p) (void)hipFree(p);
+        for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// Chunks of at most `cap` slots, cut at tile boundaries (a pixel's samples
+// stay in one chunk).
+static void batch_chunks(SheenBatch* b, size_t cap) {
+    b->chunks.clear();
+    uint32_t start = 0, len = 0;
+    for (int k = 0; k < b->n; k++) {
+        const uint32_t unit = 64u * (uint32_t)b->frames[k].ns;
+        uint32_t left = b->frames[k].n_slots;
+        while (left > 0) {
+            uint32_t room = (uint32_t)cap - len;
+            if (room < unit) {
+                b->chunks.push_back({start, len});
+                start += len;
+                len = 0;
+                room = (uint32_t)cap;
+            }
+            const uint32_t take = std::min(left, room / unit * unit);
+            len += take;
+            left -= take;
+        }
+    }
+    if (len > 0) b->chunks.push_back({start, len});
+}
+
+// The batch frame's kernels on the batch stream (captured into the graph).
+static void batch_enqueue(SheenBatch* b) {
+    hipStream_t s = b->stream;
+    WorkBufs& w = b->wb;
+    const int cu = b->num_cu;
+    k_zero_counters<<<1, 64, 0, s>>>(b->ctr);
+    EventBufs EB;
+    EB.base = w.ev_base; EB.caustic = nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
+    EB.shadow_q = w.shadow_q; EB.shadow_n = &b->ctr->shadow_n; EB.L = b->L;
+    SampleBufs SB;
+    SB.hit_p = w.hit_p; SB.samp = w.samp; SB.beam_add = nullptr; SB.media_p = nullptr; SB.media_r = nullptr; SB.n_beam = 0;
+    for (const auto& ch : b->chunks) {
+        ManySrc src;
+        src.S = b->d_scenes; src.F = b->d_frames; src.base = b->d_base; src.n = b->n;
+        src.g0 = ch.first; src.count = ch.second;
+        const size_t n = ch.second;
+        k_reset<<<1, 64, 0, s>>>(b->ctr);
+        k_primary<false, false><<<grid_n(cu, n, b->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, b->ctr);
+        k_shade_primary<false><<<grid_n(cu, n, b->shade_grid), TRACE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[0], b->ctr);
+        for (int round = 0; round < 3; round++) {
+            const int in_q = round & 1, out_q = (round + 1) & 1;
+            k_trace_sec<false, false><<<grid_n(cu, 2 * n, b->sec_grid), TRACE_BLOCK, 0, s>>>(src, w.sec_q[in_q], round,
+                                                                                            w.sec_hits, b->ctr);
+            k_shade_sec<false><<<grid_n(cu, 2 * n, b->shade_sec_grid), TRACE_BLOCK, 0, s>>>(
+                src, SB, EB, w.sec_q[in_q], round, w.sec_hits, w.sec_q[out_q], round + 1, b->ctr);
+        }
+        k_shadow<false, false><<<grid_n(cu, 3 * n * (size_t)b->L, b->shadow_grid), TRACE_BLOCK, 0, s>>>(src, EB, b->ctr);
+        k_accumulate<<<grid_n(cu, n, 8), 256, 0, s>>>(src, SB, EB, b->frame);
+    }
+}
+
+static void batch_stats(SheenRayStats* S, const Counters& h, uint64_t samples) {
+    S->samples = samples;
+    S->primary_rays = samples;
+    S->shadow_rays = h.shadow_passes + h.extra_shadow;
+    S->reflection_rays = h.reflection;
+    S->refraction_rays = h.refraction;
+    S->total_rays = S->primary_rays + S->shadow_rays + S->reflection_rays + S->refraction_rays;
+}
+
+extern "C" {
+
+SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* widths, const int* heights,
+                               const int* aa_samples, int count) {
+    if (!scenes || !widths || !heights || !aa_samples || count <= 0) { sheen_set_error("bad batch arguments"); return nullptr; }
+    std::unique_ptr<SheenBatch> b(new SheenBatch());
+    b->n = count;
+    HIP_OR(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), nullptr);
+    HIP_OR(hipEventCreate(&b->ev[0]), nullptr);
+    HIP_OR(hipEventCreate(&b->ev[1]), nullptr);
+    uint64_t slot0 = 0;
+    size_t px0 = 0;
+    for (int k = 0; k < count; k++) {
+        const int W = widths[k], H = heights[k], A = aa_samples[k];
+        if (W <= 0 || H <= 0 || A < 1 || A > 64) { sheen_set_error("bad render params for batch scene %d", k); return nullptr; }
+        SheenContext* c = context_create(scenes[k]);
+        if (!c) return nullptr;
+        b->ctx.push_back(c);
+        if (c->full || (c->ds.pm.photons && c->ds.caustics > 0.0f)) {
+            sheen_set_error("batch scene %d: batches take spheres, triangles and planes without caustics", k);
+            return nullptr;
+        }
+        c->wbp = &b->serial_wb;
+        b->num_cu = c->num_cu;
+        b->L = std::max(b->L, c->ds.num_lights);
+        FrameDesc F;
+        F.width = W; F.height = H; F.ns = A; F.tiles_x = (W + 7) / 8;
+        const uint64_t slots = (uint64_t)F.tiles_x * ((H + 7) / 8) * A * 64;
+        if (slot0 + slots > 0xFFFFFFFFull) { sheen_set_error("batch exceeds 2^32 sample slots"); return nullptr; }
+        F.g0 = (uint32_t)slot0;
+        F.n_slots = (uint32_t)slots;
+        F.out0 = (uint32_t)px0;
+        b->frames.push_back(F);
+        b->base.push_back((uint32_t)slot0);
+        slot0 += slots;
+        px0 += (size_t)W * H;
+        b->samples += (uint64_t)W * H * A;
+    }
+    b->base.push_back((uint32_t)slot0);
+    b->pixels = px0;
+    std::vector<DScene> table;
+    for (SheenContext* c : b->ctx) table.push_back(c->ds);
+    if (!alloc_dev((void**)&b->d_scenes, sizeof(DScene) * count) || !alloc_dev((void**)&b->d_frames, sizeof(FrameDesc) * count) ||
+        !alloc_dev((void**)&b->d_base, sizeof(uint32_t) * (count + 1)) || !alloc_dev((void**)&b->frame, sizeof(float4) * px0) ||
+        !alloc_dev((void**)&b->ctr, sizeof(Counters))) {
+        sheen_set_error("batch allocation failed");
+        return nullptr;
+    }
+    HIP_OR(hipMemcpy(b->d_scenes, table.data(), sizeof(DScene) * count, hipMemcpyHostToDevice), nullptr);
+    HIP_OR(hipMemcpy(b->d_frames, b->frames.data(), sizeof(FrameDesc) * count, hipMemcpyHostToDevice), nullptr);
+    HIP_OR(hipMemcpy(b->d_base, b->base.data(), sizeof(uint32_t) * (count + 1), hipMemcpyHostToDevice), nullptr);
+    b->chunk_slots = std::min((size_t)16 << 20, ((size_t)8 << 30) / slot_bytes(b->L, 0, false, false));
+    batch_chunks(b.get(), b->chunk_slots);
+    if (!ensure_bufs(b->wb, b->chunk_slots, b->L, 0, false, false)) return nullptr;
+    return b.release();
+}
+
+void sheen_batch_destroy(SheenBatch* b) { delete b; }
+
+int sheen_batch_set_param(SheenBatch* b, const char* name, int value) {
+    if (!b || !name) { sheen_set_error("null argument"); return -1; }
+    if (value < 0) { sheen_set_error("negative parameter value"); return -1; }
+    const std::string n(name);
+    if (n == "primary_grid") b->primary_grid = value;
+    else if (n == "sec_grid") b->sec_grid = value;
+    else if (n == "shadow_grid") b->shadow_grid = value;
+    else if (n == "shade_grid") b->shade_grid = value;
+    else if (n == "shade_sec_grid") b->shade_sec_grid = value;
+    else if (n == "graph") b->use_graph = value != 0;
+    else if (n == "chunk") {
+        const size_t cap = value > 0 ? (size_t)value : std::min((size_t)16 << 20, ((size_t)8 << 30) / slot_bytes(b->L, 0, false, false));
+        size_t unit = 64;
+        for (const FrameDesc& F : b->frames) unit = std::max(unit, (size_t)64 * F.ns);
+        if (cap < unit) { sheen_set_error("chunk smaller than a tile of the batch"); return -1; }
+        if (!ensure_bufs(b->wb, cap, b->L, 0, false, false)) return -1;
+        b->chunk_slots = cap;
+        batch_chunks(b, cap);
+    } else { sheen_set_error("unknown parameter: %s", name); return -1; }
+    b->dirty = true;
+    return 0;
+}
+
+int sheen_batch_render(SheenBatch* b, SheenRayStats* stats) {
+    if (!b) { sheen_set_error("null batch"); return -1; }
+    hipStream_t s = b->stream;
+    if (b->use_graph && (b->dirty || !b->exec)) {
+        if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+        if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+        HIP_OR(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), -1);
+        batch_enqueue(b);
+        HIP_OR(hipStreamEndCapture(s, &b->graph), -1);
+        HIP_OR(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), -1);
+        b->dirty = false;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_OR(hipEventRecord(b->ev[0], s), -1);
+    if (b->use_graph) HIP_OR(hipGraphLaunch(b->exec, s), -1);
+    else batch_enqueue(b);
+    HIP_OR(hipEventRecord(b->ev[1], s), -1);
+    HIP_OR(hipGetLastError(), -1);
+    Counters host{};
+    HIP_OR(hipMemcpyAsync(&host, b->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s), -1);
+    HIP_OR(hipStreamSynchronize(s), -1);
+    const auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        SheenRayStats S{};
+        batch_stats(&S, host, b->samples);
+        HIP_OR(hipEventElapsedTime(&S.frame_ms, b->ev[0], b->ev[1]), -1);
+        S.trace_ms = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();  // wall clock of the call
+        S.shadow_launches = (int)b->chunks.size();
+        *stats = S;
+    }
+    return 0;
+}
+
+int sheen_batch_render_serial(SheenBatch* b, SheenRayStats* stats) {
+    if (!b) { sheen_set_error("null batch"); return -1; }
+    SheenRayStats T{};
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < b->n; k++) {
+        SheenRayStats S{};
+        const FrameDesc& F = b->frames[k];
+        if (render_frame(b->ctx[k], F.width, F.height, F.ns, &S, false, nullptr, nullptr, nullptr) != 0) return -1;
+        T.samples += S.samples; T.primary_rays += S.primary_rays; T.shadow_rays += S.shadow_rays;
+        T.reflection_rays += S.reflection_rays; T.refraction_rays += S.refraction_rays; T.total_rays += S.total_rays;
+        T.frame_ms += S.frame_ms;
+        T.shadow_launches += S.shadow_launches;
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    T.trace_ms = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (stats) *stats = T;
+    return 0;
+}
+
+int sheen_batch_image(SheenBatch* b, int index, int serial, float* out_rgba) {
+    if (!b || !out_rgba || index < 0 || index >= b->n) { sheen_set_error("bad batch image request"); return -1; }
+    const FrameDesc& F = b->frames[index];
+    const size_t bytes = (size_t)F.width * F.height * sizeof(float4);
+    if (serial) {
+        SheenContext* c = b->ctx[index];
+        if (!c->frame || c->frame_w != F.width || c->frame_h != F.height) { sheen_set_error("scene not rendered serially"); return -1; }
+        HIP_OR(hipMemcpy(out_rgba, c->frame, bytes, hipMemcpyDeviceToHost), -1);
+    } else {
+        HIP_OR(hipStreamSynchronize(b->stream), -1);
+        HIP_OR(hipMemcpy(out_rgba, b->frame + F.out0, bytes, hipMemcpyDeviceToHost), -1);
+    }
+    return 0;
+}
+
+int sheen_batch_info(SheenBatch* b, int* scenes, long long* samples, int* chunks) {
+    if (!b) { sheen_set_error("null batch"); return -1; }
+    if (scenes) *scenes = b->n;
+    if (samples) *samples = (long long)b->samples;
+    if (chunks) *chunks = (int)b->chunks.size();
+    return 0;
 }
 
 int sheen_render_float(const SheenScene* scene, int width, int height, int aa_samples, float* out_rgba,

@@ -62,6 +62,7 @@ struct Store {
     int photons = 0;
     float photon_cell = 0.0f, caustics = 0.0f;
     CameraData cam{};
+    int view_w = 0, view_h = 0, view_aa = 0;  // a generator's own image size and spp (0 = the params')
 
     void finalize() {
         SheenScene& s = scene;
@@ -349,6 +350,44 @@ void gen_cfg5(Store& st, const SheenSceneParams& p, Rng& rng) {
         float inten = rng.uni(0.5f, 2.0f);
         st.sun_light(dir, f3(1, 1, 1), inten * sc, 0.0f);
     }
+}
+
+// One scene of the uneven batch workload: triangle count log-uniform in
+// [1e3, 2e5] (a displaced geodesic sphere; detail > 0 fixes it), 4-32 random
+// spheres, a ground plane, a point light and a sun; square resolution drawn
+// from {128, 256, 512, 1024} and 1-8 spp, all from the seed.
+void gen_cfg4(Store& st, const SheenSceneParams& p, Rng& rng) {
+    const float lo = std::log(1.0e3f), hi = std::log(2.0e5f);
+    const double tris = p.detail > 0 ? (double)p.detail : std::exp((double)rng.uni(lo, hi));
+    const int f = std::max(1, (int)std::lround(std::sqrt(tris / 20.0)));
+    static const int res[4] = {128, 256, 512, 1024};
+    st.view_w = st.view_h = res[rng.next() % 4];
+    st.view_aa = 1 + (int)(rng.next() % 8);
+    const float az = rng.uni(0.0f, 6.2831853f);
+    st.cam = look_at(f3(3.4f * std::sin(az), rng.uni(0.3f, 1.5f), -3.4f * std::cos(az)), f3(0, 0, 0), 45.0f);
+    std::vector<int> mats;
+    for (int k = 0; k < 4; k++) {
+        sheen_f3 alb = f3(rng.uni(0.15f, 0.9f), rng.uni(0.15f, 0.9f), rng.uni(0.15f, 0.9f));
+        mats.push_back(st.material(alb, 0.4f, 48.0f, 0.08f));
+    }
+    int mirror = st.material(f3(0.9f, 0.9f, 0.9f), 0.6f, 128.0f, 0.02f);
+    st.mats[mirror].reflect = 0.9f;
+    int glass = st.material(f3(0.95f, 0.95f, 0.95f), 0.8f, 256.0f, 0.0f);
+    st.mats[glass].transmission = 0.9f; st.mats[glass].reflect = 0.1f; st.mats[glass].ior = 1.5f;
+    geodesic_mesh(st, f, 1.0f, 0.12f, (uint32_t)(p.seed ^ (p.seed >> 32)), mats, mirror, glass, rng);
+    const int ns = 4 + (int)(rng.next() % 29);
+    for (int i = 0; i < ns; i++) {
+        const float r = rng.uni(0.08f, 0.3f);
+        const float a = rng.uni(0.0f, 6.2831853f), d = rng.uni(1.4f, 3.0f);
+        sheen_f3 alb = f3(rng.uni(0.1f, 0.9f), rng.uni(0.1f, 0.9f), rng.uni(0.1f, 0.9f));
+        int m = st.material(alb, 0.3f, 32.0f, 0.05f);
+        if (rng.uni() < 0.2f) st.mats[m].reflect = 0.6f;
+        st.sphere(f3(d * std::cos(a), -1.2f + r, d * std::sin(a)), r, m);
+    }
+    int g = st.material(f3(0.6f, 0.6f, 0.6f), 0.1f, 16.0f, 0.1f);
+    st.plane(f3(0, 1, 0), 1.2f, g);
+    st.point_light(f3(rng.uni(-3.0f, 3.0f), 3.5f, rng.uni(-3.0f, 3.0f)), f3(1.0f, 0.95f, 0.9f), 250.0f, 0.0f);
+    st.sun_light(f3(rng.uni(-0.6f, 0.6f), -1.0f, rng.uni(-0.6f, 0.6f)), f3(1, 1, 1), 0.8f, 0.0f);
 }
 
 // Parity scene 100: every light type, glass and mirror spheres, soft shadows.
@@ -709,6 +748,7 @@ int sheen_scene_defaults(int config, SheenSceneParams* out) {
     switch (config) {
         case 1: out->width = 256; out->height = 256; out->aa = 1; out->detail = 64; break;
         case 2: out->width = 1920; out->height = 1080; out->aa = 4; out->detail = 229; break;
+        case 4: out->width = 256; out->height = 256; out->aa = 4; out->detail = 0; break;
         case 5: out->width = 512; out->height = 512; out->aa = 8; out->detail = 10000; out->scale = 1.0f; break;
         case 100: case 101: case 102: out->width = 160; out->height = 120; out->aa = 4; out->detail = 0; break;
         case 103: out->width = 320; out->height = 240; out->aa = 1; out->detail = 20000; break;
@@ -729,6 +769,7 @@ void* sheen_scene_generate(const SheenSceneParams* p) {
     switch (q.config) {
         case 1: gen_cfg1(*st, q, rng); break;
         case 2: gen_cfg2(*st, q, rng); break;
+        case 4: gen_cfg4(*st, q, rng); break;
         case 5: gen_cfg5(*st, q, rng); break;
         case 100: gen_p100(*st, q, rng); break;
         case 101: gen_p101(*st, q, rng); break;
@@ -744,6 +785,15 @@ void* sheen_scene_generate(const SheenSceneParams* p) {
 }
 
 const SheenScene* sheen_scene_get(void* h) { return h ? &((Store*)h)->scene : nullptr; }
+
+int sheen_scene_view(void* h, int* width, int* height, int* aa) {
+    Store* s = (Store*)h;
+    if (!s || !s->view_w) return 0;
+    if (width) *width = s->view_w;
+    if (height) *height = s->view_h;
+    if (aa) *aa = s->view_aa;
+    return 1;
+}
 
 Material* sheen_scene_materials(void* h, int* count) {
     Store* s = (Store*)h;

@@ -66,6 +66,13 @@ def capi():
     _sig(lib, "sheen_context_set_counting", C.c_int, [P, C.c_int])
     _sig(lib, "sheen_context_set_chunk", C.c_int, [P, C.c_int])
     _sig(lib, "sheen_context_set_param", C.c_int, [P, C.c_char_p, C.c_int])
+    _sig(lib, "sheen_batch_create", P, [P, P, P, P, C.c_int])
+    _sig(lib, "sheen_batch_destroy", None, [P])
+    _sig(lib, "sheen_batch_set_param", C.c_int, [P, C.c_char_p, C.c_int])
+    _sig(lib, "sheen_batch_render", C.c_int, [P, C.POINTER(SheenRayStats)])
+    _sig(lib, "sheen_batch_render_serial", C.c_int, [P, C.POINTER(SheenRayStats)])
+    _sig(lib, "sheen_batch_image", C.c_int, [P, C.c_int, C.c_int, P])
+    _sig(lib, "sheen_batch_info", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)])
     _sig(lib, "sheen_adjoint_sizes", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)])
     _sig(lib, "sheen_adjoint_get_params", C.c_int, [P, P, P, P, P, P])
     _sig(lib, "sheen_adjoint_forward", C.c_int, [P, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P])
@@ -92,6 +99,7 @@ def host():
     _sig(lib, "sheen_scene_defaults", C.c_int, [C.c_int, C.POINTER(SheenSceneParams)])
     _sig(lib, "sheen_scene_generate", P, [C.POINTER(SheenSceneParams)])
     _sig(lib, "sheen_scene_get", C.POINTER(contract.SheenScene), [P])
+    _sig(lib, "sheen_scene_view", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)])
     _sig(lib, "sheen_scene_materials", C.POINTER(contract.Material), [P, C.POINTER(C.c_int)])
     _sig(lib, "sheen_scene_lights", C.POINTER(contract.Light), [P, C.POINTER(C.c_int)])
     _sig(lib, "sheen_scene_texture", C.POINTER(C.c_ubyte), [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)])

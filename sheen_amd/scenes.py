@@ -3,7 +3,7 @@ import ctypes as C
 
 from . import native
 
-CONFIGS = {1: "spheres-64-256x256", 2: "mesh-1M-1080p", 5: "shading-10k-512x512",
+CONFIGS = {1: "spheres-64-256x256", 2: "mesh-1M-1080p", 4: "batch-scene", 5: "shading-10k-512x512",
            100: "parity-lights", 101: "parity-mesh", 102: "parity-textures"}
 
 
@@ -23,6 +23,9 @@ class GeneratedScene:
             raise RuntimeError("scene generation failed for %r" % config)
         self.scene = lib.sheen_scene_get(self._h)
         self.width, self.height, self.aa = p.width, p.height, p.aa
+        w, h, a = C.c_int(), C.c_int(), C.c_int()
+        if lib.sheen_scene_view(self._h, C.byref(w), C.byref(h), C.byref(a)):
+            self.width, self.height, self.aa = w.value, h.value, a.value
 
     def materials(self):
         n = C.c_int()
@@ -54,3 +57,13 @@ class GeneratedScene:
 
 def generate(config, **overrides):
     return GeneratedScene(config, **overrides)
+
+
+BATCH_SEED = 0x5EED0400
+
+
+def batch(n=256, seed=BATCH_SEED, **overrides):
+    """The uneven batch workload: n config-4 scenes with seeds seed..seed+n-1
+    (triangle counts log-uniform in [1e3, 2e5], resolutions from
+    {128, 256, 512, 1024}, 1-8 spp)."""
+    return [GeneratedScene(4, seed=seed + k, **overrides) for k in range(n)]

@@ -1,0 +1,80 @@
+"""Batches of scenes in one wavefront pipeline (sheen_batch_*): every scene
+of a batch renders as it does alone, for any chunking, with or without the
+captured hipGraph."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# uneven sizes (not multiples of the 8x8 tile), spp 1..8, 1 or 2 lights
+SIZES = [(64, 64, 1), (120, 72, 3), (40, 88, 8), (96, 96, 2), (33, 17, 5), (128, 64, 4), (72, 72, 6), (16, 16, 1)]
+
+
+@pytest.fixture(scope="module")
+def batch():
+    from sheen_amd import scenes
+    from sheen_amd.batch import Batch
+    sc = [scenes.generate(4, seed=scenes.BATCH_SEED + k, detail=(1500 if k % 2 else 12000)) for k in range(6)]
+    sc.append(scenes.generate(1))    # 64 spheres + plane, one light
+    sc.append(scenes.generate(101))  # meshes, cubes, glass
+    b = Batch(sc, SIZES)
+    yield {"b": b, "scenes": sc}
+    b.close()
+
+
+def test_batched_images_equal_serial_renders(batch):
+    b = batch["b"]
+    s_serial = b.render_serial()
+    s_batch = b.render()
+    for k in range(b.n):
+        assert np.array_equal(b.image(k), b.image(k, serial=True)), k
+    for key in ("samples", "primary_rays", "shadow_rays", "reflection_rays", "refraction_rays", "total_rays"):
+        assert s_batch[key] == s_serial[key], key
+
+
+def test_graph_replays_match_direct_launches(batch):
+    b = batch["b"]
+    b.set_param("graph", 0)
+    b.render()
+    ref = [b.image(k) for k in range(b.n)]
+    b.set_param("graph", 1)
+    for _ in range(3):
+        b.render_serial()  # other work on the device between replays
+        b.render()
+        for k in range(b.n):
+            assert np.array_equal(b.image(k), ref[k]), k
+
+
+def test_chunks_across_scene_boundaries(batch):
+    b = batch["b"]
+    b.set_param("chunk", 0)
+    b.render()
+    ref = [b.image(k) for k in range(b.n)]
+    b.set_param("chunk", 64 * 8 * 7)  # 7 tiles of 8 spp: chunks straddle scenes
+    assert b.chunks > b.n
+    b.render()
+    for k in range(b.n):
+        assert np.array_equal(b.image(k), ref[k]), k
+    b.set_param("chunk", 0)
+
+
+def test_batched_scene_matches_oracle(batch):
+    from oracle import Oracle
+    b = batch["b"]
+    b.render()
+    k = 6  # the 64-sphere scene
+    w, h, a = SIZES[k]
+    ref = Oracle(batch["scenes"][k].scene, threads=min(16, os.cpu_count() or 1)).render(w, h, a)["image"]
+    img = b.image(k)
+    rel = np.abs(img[..., :3] - ref[..., :3]) / np.maximum(np.abs(ref[..., :3]), 1e-3)
+    assert rel.max() <= 1e-4
+
+
+def test_unsupported_scene_is_refused():
+    from sheen_amd import scenes
+    from sheen_amd.batch import Batch
+    sc = scenes.generate(104)
+    with pytest.raises(RuntimeError, match="batches take spheres, triangles and planes"):
+        Batch([sc], [(32, 32, 1)])
