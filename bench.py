@@ -8,8 +8,10 @@ glass, 2 point lights + 1 area light), traced through the full reference path
 is every traversal call of the timed frames divided by their wall time, with
 the scene already resident in HBM.
 
-Also measured (secondary, in the same JSON line): "adjoint", forward and
-backward time of the differentiable-shading workload (adjoint / forward).
+Also measured (secondary, in the same JSON line): "batch", the 256-scene
+uneven batch as one pipeline against one render per scene (speedup), and
+"adjoint", forward and backward time of the differentiable-shading workload
+(adjoint / forward).
 
 Prints one JSON line (see --help).  --gpus above 1 is refused: the path runs
 on one GPU.
@@ -95,6 +97,33 @@ def adjoint_leg(steps):
             "parameters": {k: int(v.numel()) for k, v in p.items()}}
 
 
+def batch_leg(steps):
+    """The uneven batch: 256 scenes (triangle counts log-uniform in
+    [1e3, 2e5], square images of 128-1024 pixels, 1-8 spp) rendered as one
+    wavefront pipeline replayed from a hipGraph, against one render per
+    scene through the same resident contexts (wall clock of each call)."""
+    from sheen_amd import scenes
+    from sheen_amd.batch import Batch
+    sc = scenes.batch(256)
+    b = Batch(sc)
+    b.render()
+    b.render_serial()
+    bt = st = 0.0
+    rays = 0
+    for _ in range(steps):
+        s = b.render()
+        bt += s["trace_ms"] / steps
+        rays = s["total_rays"]
+    for _ in range(steps):
+        st += b.render_serial()["trace_ms"] / steps
+    out = {"workload": "batch-256: %d scenes, %.1fM triangles, %.0fM samples" %
+                       (b.n, sum(s.scene.contents.num_triangles for s in sc) / 1e6, b.samples / 1e6),
+           "batched_ms": round(bt, 2), "serial_ms": round(st, 2), "speedup": round(st / bt, 3),
+           "batched_mrays_s": round(rays / bt / 1e3, 1), "chunks": b.chunks}
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +131,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-shading leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-scene batch leg")
     ap.add_argument("--detail", type=int, default=0, help="geodesic frequency override (default 229)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -195,6 +225,8 @@ def main():
                                                                         "shade_ms")},
     }
     ctx.close()
+    if not args.no_batch:
+        line["batch"] = batch_leg(max(2, min(args.steps, 3)))
     if not args.no_adjoint:
         line["adjoint"] = adjoint_leg(max(3, min(args.steps, 10)))
     if not args.no_cpu_baseline:

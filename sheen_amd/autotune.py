@@ -101,3 +101,40 @@ def save(result, path):
 def load(path):
     with open(path) as f:
         return json.load(f)["params"]
+
+
+# Batches: the knobs of one frame of every scene, scored by the batch's GPU
+# frame time (the kernels are replayed from one captured graph, so per-kernel
+# timers do not apply).
+BATCH_SPACE = {
+    "chunk": (1 << 23, 1 << 24, 1 << 25),
+    "shadow_grid": (16, 32, 64, 128),
+    "primary_grid": (8, 16, 64, 0),
+    "sec_grid": (4, 8, 16, 32),
+    "shade_grid": (1, 2, 4, 8),
+    "shade_sec_grid": (1, 2, 4, 8),
+}
+
+
+def tune_batch(batch, space=None, reps=2, log=None):
+    """Coordinate descent over the batch knobs; leaves the batch configured
+    with the best values.  Returns {"params", "frame_ms", "trials"}."""
+    space = dict(BATCH_SPACE if space is None else space)
+    params = {}
+    trials = []
+
+    def measure():
+        batch.render()  # first frame at a new shape recaptures the graph
+        return min(batch.render()["frame_ms"] for _ in range(reps))
+
+    for name, values in space.items():
+        scores = {}
+        for v in values:
+            batch.set_param(name, v)
+            scores[v] = measure()
+            trials.append({"param": name, "value": v, "frame_ms": round(scores[v], 3)})
+            if log:
+                log(trials[-1])
+        params[name] = min(scores, key=scores.get)
+        batch.set_param(name, params[name])
+    return {"params": params, "frame_ms": measure(), "trials": trials}

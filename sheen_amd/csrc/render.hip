@@ -923,7 +923,7 @@ struct SheenBatch {
     WorkBufs wb, serial_wb;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    int primary_grid = 64, sec_grid = 6, shadow_grid = 64, shade_grid = 2, shade_sec_grid = 2;
+    int primary_grid = 64, sec_grid = 4, shadow_grid = 128, shade_grid = 2, shade_sec_grid = 2;  // tuned on the 256-scene batch
     size_t chunk_slots = 0;
     bool use_graph = true, dirty = true;
     hipGraph_t graph = nullptr;
@@ -1057,7 +1057,9 @@ SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* width
     HIP_OR(hipMemcpy(b->d_scenes, table.data(), sizeof(DScene) * count, hipMemcpyHostToDevice), nullptr);
     HIP_OR(hipMemcpy(b->d_frames, b->frames.data(), sizeof(FrameDesc) * count, hipMemcpyHostToDevice), nullptr);
     HIP_OR(hipMemcpy(b->d_base, b->base.data(), sizeof(uint32_t) * (count + 1), hipMemcpyHostToDevice), nullptr);
-    b->chunk_slots = std::min((size_t)16 << 20, ((size_t)8 << 30) / slot_bytes(b->L, 0, false, false));
+    // 32M-slot chunks (within 16 GiB of work buffers) measured fastest on the 256-scene batch
+    b->chunk_slots = std::min((size_t)32 << 20, ((size_t)16 << 30) / slot_bytes(b->L, 0, false, false));
+    b->chunk_slots = std::min<size_t>(b->chunk_slots, ((size_t)slot0 + 63) / 64 * 64 + 64 * 64);
     batch_chunks(b.get(), b->chunk_slots);
     if (!ensure_bufs(b->wb, b->chunk_slots, b->L, 0, false, false)) return nullptr;
     return b.release();
@@ -1076,7 +1078,7 @@ int sheen_batch_set_param(SheenBatch* b, const char* name, int value) {
     else if (n == "shade_sec_grid") b->shade_sec_grid = value;
     else if (n == "graph") b->use_graph = value != 0;
     else if (n == "chunk") {
-        const size_t cap = value > 0 ? (size_t)value : std::min((size_t)16 << 20, ((size_t)8 << 30) / slot_bytes(b->L, 0, false, false));
+        const size_t cap = value > 0 ? (size_t)value : std::min((size_t)32 << 20, ((size_t)16 << 30) / slot_bytes(b->L, 0, false, false));
         size_t unit = 64;
         for (const FrameDesc& F : b->frames) unit = std::max(unit, (size_t)64 * F.ns);
         if (cap < unit) { sheen_set_error("chunk smaller than a tile of the batch"); return -1; }
