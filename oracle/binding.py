@@ -61,6 +61,7 @@ def lib():
     L.oracle_photon_probe_brute.restype = C.c_int
     L.oracle_photon_probe_brute.argtypes = [P, C.c_int, C.c_float, C.c_int, P]
     L.oracle_set_params.argtypes = [P, P, P, P, P]
+    L.oracle_render_reference.argtypes = [P] + [C.c_int] * 10 + [P]
     _lib = L
     return L
 
@@ -134,6 +135,17 @@ class Oracle:
         f = self._L.oracle_photon_probe_brute if brute else self._L.oracle_photon_probe
         k = f(self._h, stride, radius, max_samples, _p(out))
         return out[:k]
+
+    def render_reference(self, width, height, window, sub=8, aa=64, fp64=True):
+        """Radiance of window (x0, y0, w, h) of the width x height image with
+        sub*sub*aa samples per pixel over its footprint; (h, w, 3) float64."""
+        x0, y0, cw, ch = window
+        out = np.zeros((ch, cw, 3), np.float64)
+        rc = self._L.oracle_render_reference(self._h, width, height, x0, y0, cw, ch, sub, aa, self.threads, int(fp64),
+                                             _p(out))
+        if rc != 0:
+            raise RuntimeError("oracle reference render failed")
+        return out
 
     def set_params(self, albedo=None, emission=None, intensity=None, texels=None):
         """Substitutes the differentiable parameters (float32 arrays: albedo and

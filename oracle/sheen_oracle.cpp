@@ -2016,6 +2016,50 @@ int oracle_render(void* h, int width, int height, int aa, int y0, int y1, int th
     return 0;
 }
 
+// Reference radiance of a window of the width x height image: every pixel
+// averages its sub x sub sub-pixels of the (sub*width) x (sub*height) image,
+// each rendered with aa samples (FP64 arithmetic, or the FP32 parity path),
+// i.e. sub*sub*aa samples spread over the pixel's footprint (4096 with
+// sub = 8, aa = 64).  out: double RGB, cw*ch*3, row-major over the window.
+int oracle_render_reference(void* h, int width, int height, int x0, int y0, int cw, int ch, int sub, int aa,
+                            int threads, int fp64, double* out_rgb) {
+    Scene* S = (Scene*)h;
+    if (!S || width <= 0 || height <= 0 || sub < 1 || aa < 1 || aa > 64 || cw <= 0 || ch <= 0 || !out_rgb) return -1;
+    if (x0 < 0 || y0 < 0 || x0 + cw > width || y0 + ch > height) return -1;
+    if (threads < 1) threads = 1;
+    const int W = width * sub, H = height * sub;
+#pragma omp parallel num_threads(threads)
+    {
+        Counters c;
+#pragma omp for schedule(dynamic, 1) collapse(2)
+        for (int j = 0; j < ch; j++) {
+            for (int i = 0; i < cw; i++) {
+                double acc[3] = {0, 0, 0};
+                for (int sy = 0; sy < sub; sy++) {
+                    for (int sx = 0; sx < sub; sx++) {
+                        const int X = (x0 + i) * sub + sx, Y = (y0 + j) * sub + sy;
+                        for (int s = 0; s < aa; s++) {
+                            int hk, hi;
+                            float ht;
+                            if (fp64) {
+                                V3<double> col = render_sample<double>(*S, X, Y, s, aa, W, H, c, &hk, &hi, &ht);
+                                acc[0] += col.x; acc[1] += col.y; acc[2] += col.z;
+                            } else {
+                                V3<float> col = render_sample<float>(*S, X, Y, s, aa, W, H, c, &hk, &hi, &ht);
+                                acc[0] += col.x; acc[1] += col.y; acc[2] += col.z;
+                            }
+                        }
+                    }
+                }
+                const double inv = 1.0 / (double(sub) * sub * aa);
+                double* o = out_rgb + ((size_t)j * cw + i) * 3;
+                o[0] = acc[0] * inv; o[1] = acc[1] * inv; o[2] = acc[2] * inv;
+            }
+        }
+    }
+    return 0;
+}
+
 // Closest hit of arbitrary rays (scene_intersect).  For per-primitive tests.
 int oracle_trace(void* h, int n, const float* orig, const float* dir, float tmin, float tmax, int opaque,
                  int* kind, int* index, float* t, float* point, float* normal, int* material) {
