@@ -65,6 +65,19 @@ unsigned char* sheen_scene_texture(void* handle, int tex, int* w, int* h);
 
 void sheen_scene_free(void* handle);
 
+/* ---- .sheen scene files (reference api/scene_io.cpp) -----------------------
+ * sheen_file_load parses a .sheen file (curves tessellated on the host) and
+ * always returns a handle; sheen_file_error is "" on success, else the
+ * message with its line number.  The scene and the output path stay valid
+ * until sheen_file_free.  Relative file names inside the scene are tried
+ * as given, then next to the scene file. */
+typedef struct SheenFile SheenFile;
+SheenFile* sheen_file_load(const char* path);
+const char* sheen_file_error(const SheenFile* file);
+const SheenScene* sheen_file_scene(const SheenFile* file);
+int sheen_file_settings(const SheenFile* file, int* width, int* height, int* aa, const char** output);
+void sheen_file_free(SheenFile* file);
+
 #ifdef __cplusplus
 }
 #endif

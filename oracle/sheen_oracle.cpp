@@ -1850,6 +1850,7 @@ void* oracle_scene_create(const SheenScene* in, int threads) {
         std::vector<int> idx;
         for (int i = 0; i < counts[c]; i++) {
             if (c == K_SPHERE && in->objects[i].type != 0) continue;
+            if (c == K_SE && in->superellipsoids[i]._pad0 == 1) continue;  // CSG-only (scene.h:99-104 se_hidden)
             idx.push_back(i);
         }
         S->count[c] = counts[c];

@@ -45,7 +45,7 @@ def build(force=True):
     os.makedirs(OBJ, exist_ok=True)
     hdrs = _headers()
     objs = []
-    for src in HOST_SRCS + ["scenes.cpp", "host_api.cpp"]:
+    for src in HOST_SRCS + ["scenes.cpp", "host_api.cpp", "sheen_file.cpp"]:
         o = os.path.join(OBJ, src.replace(".cpp", ".o"))
         if force or _newer(o, [os.path.join(CSRC, src)] + hdrs):
             _run(["g++"] + CXXFLAGS + ["-I", INC, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", o])

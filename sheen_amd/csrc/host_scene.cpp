@@ -179,6 +179,8 @@ bool host_prepare_scene(const SheenScene& s, HostScene& out, std::string& err) {
         for (int i = 0; i < counts[c]; i++) {
             host_prim_bounds(s, c, i, &hb.bounds[6 * (size_t)i], &hb.bounds[6 * (size_t)i + 3]);
             if (c == 0 && s.objects[i].type != 0) { skip[i] = 1; continue; }
+            // a superellipsoid flagged _pad0 = 1 is a CSG operand only (.sheen se_csg)
+            if (c == 4 && s.superellipsoids[i]._pad0 == 1) { skip[i] = 1; continue; }
             maxtr = std::fmax(maxtr, s.materials[prim_material(s, c, i)].transmission);
         }
         hb.opaque = maxtr <= 0.5f ? 1 : 0;

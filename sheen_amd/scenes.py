@@ -67,3 +67,43 @@ def batch(n=256, seed=BATCH_SEED, **overrides):
     (triangle counts log-uniform in [1e3, 2e5], resolutions from
     {128, 256, 512, 1024}, 1-8 spp)."""
     return [GeneratedScene(4, seed=seed + k, **overrides) for k in range(n)]
+
+
+class SheenFile:
+    """A parsed .sheen scene file: ``.scene`` (SheenScene pointer), ``.width``,
+    ``.height``, ``.aa``, ``.output``.  Raises ValueError with the loader's
+    message (and line number) when the file does not parse."""
+
+    def __init__(self, path):
+        lib = native.host()
+        self._h = lib.sheen_file_load(str(path).encode())
+        err = lib.sheen_file_error(self._h).decode()
+        if err:
+            lib.sheen_file_free(self._h)
+            self._h = None
+            raise ValueError(err)
+        self.scene = lib.sheen_file_scene(self._h)
+        w, h, a, out = C.c_int(), C.c_int(), C.c_int(), C.c_char_p()
+        lib.sheen_file_settings(self._h, C.byref(w), C.byref(h), C.byref(a), C.byref(out))
+        self.width, self.height, self.aa, self.output = w.value, h.value, a.value, out.value.decode()
+
+    def close(self):
+        if self._h:
+            native.host().sheen_file_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def load_sheen(path):
+    return SheenFile(path)
