@@ -150,11 +150,30 @@ typedef struct SheenRayStats {
 
 typedef struct SheenContext SheenContext;
 
-/* Builds the wide BVHs on the host and uploads the scene to the current
- * device.  The scene arrays are copied; the caller keeps ownership.
- * Returns NULL with sheen_get_last_error() set on failure. */
+/* Uploads the scene to the current device and builds its wide BVHs there
+ * (binned SAH, one BVH per primitive kind); replaces the reference's host
+ * build per render call (api/sheen_c_api.cu:138, scene.h:410-481).  The
+ * scene arrays are copied; the caller keeps ownership.  Returns NULL with
+ * sheen_get_last_error() set on failure. */
 SheenContext* sheen_context_create(const SheenScene* scene);
 void sheen_context_destroy(SheenContext* ctx);
+
+/* sheen_context_create with options.  SHEEN_CTX_HOST_BVH: build the BVHs
+ * with the host builder instead (the same trees, byte for byte). */
+#define SHEEN_CTX_HOST_BVH 1
+SheenContext* sheen_context_create_ex(const SheenScene* scene, int flags);
+
+/* The wide BVH of one primitive kind (0 spheres, 1 strands, 2 triangles,
+ * 3 height fields, 4 superellipsoids, 5 lathes, 6 prisms).  info[4]: nodes,
+ * leaf slots, worst-case traversal stack entries, wide levels.  nodes
+ * (128-byte records, node_cap of them) and prims (primitive index per leaf
+ * slot) may be NULL to query the sizes. */
+int sheen_context_bvh(const SheenContext* ctx, int category, void* nodes, int node_cap, unsigned int* prims,
+                      int prim_cap, int* info);
+
+/* Milliseconds sheen_context_create spent on validation, bounds and BVH
+ * builds (device builds included, photon pass excluded). */
+double sheen_context_build_ms(const SheenContext* ctx);
 
 /* Renders one frame.  out_rgba: host float[width*height*4] or NULL (the
  * frame then stays in device memory, see sheen_context_frame_device).

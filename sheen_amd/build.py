@@ -18,6 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SHEEN_OFFLOAD_ARCH", "gfx950")
 
 HOST_SRCS = ["wbvh_build.cpp", "host_scene.cpp", "png.cpp"]
+HIP_SRCS = ["render.hip", "wbvh_gpu.hip"]
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fwrapv", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "--offload-arch=" + ARCH,
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
@@ -52,12 +53,15 @@ def build(force=True):
         objs.append(o)
     host_lib = os.path.join(HERE, "libsheen_host.so")
     _run(["g++", "-shared", "-o", host_lib] + objs + ["-lz"])
-    render_o = os.path.join(OBJ, "render.o")
-    if force or _newer(render_o, [os.path.join(CSRC, "render.hip")] + hdrs):
-        _run([HIPCC] + HIPFLAGS + ["-I", INC, "-I", CSRC, "-c", os.path.join(CSRC, "render.hip"), "-o", render_o])
+    hip_objs = []
+    for src in HIP_SRCS:
+        o = os.path.join(OBJ, src.replace(".hip", ".o"))
+        if force or _newer(o, [os.path.join(CSRC, src)] + hdrs):
+            _run([HIPCC] + HIPFLAGS + ["-I", INC, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", o])
+        hip_objs.append(o)
     capi = os.path.join(HERE, "libsheen_c_api.so")
     host_objs = [os.path.join(OBJ, s.replace(".cpp", ".o")) for s in HOST_SRCS]
-    _run([HIPCC, "-shared", "--offload-arch=" + ARCH, "-o", capi, render_o] + host_objs + ["-lz"])
+    _run([HIPCC, "-shared", "--offload-arch=" + ARCH, "-o", capi] + hip_objs + host_objs + ["-lz"])
     # CPython module
     pyinc = sysconfig.get_paths()["include"]
     ext = sysconfig.get_config_var("EXT_SUFFIX") or ".so"

@@ -126,7 +126,7 @@ static int prim_material(const SheenScene& s, int cat, int i) {
     return 0;
 }
 
-bool host_prepare_scene(const SheenScene& s, HostScene& out, std::string& err) {
+bool host_prepare_scene(const SheenScene& s, HostScene& out, std::string& err, bool build_bvh) {
     auto t0 = std::chrono::steady_clock::now();
     const int counts[7] = {s.objects ? s.num_objects : 0, s.sweep_strands ? s.num_sweep_strands : 0,
                            s.triangles ? s.num_triangles : 0, s.heightfields ? s.num_heightfields : 0,
@@ -173,17 +173,20 @@ bool host_prepare_scene(const SheenScene& s, HostScene& out, std::string& err) {
         HostBvh& hb = out.bvh[c];
         hb.count = counts[c];
         if (counts[c] == 0) continue;
-        hb.bounds.resize(6 * (size_t)counts[c]);
-        std::vector<unsigned char> skip(counts[c], 0);
+        // triangle bounds are computed where the BVH is built
+        const bool want_bounds = build_bvh || c != 2;
+        if (want_bounds) hb.bounds.resize(6 * (size_t)counts[c]);
+        hb.skip.assign(counts[c], 0);
         float maxtr = 0.0f;
         for (int i = 0; i < counts[c]; i++) {
-            host_prim_bounds(s, c, i, &hb.bounds[6 * (size_t)i], &hb.bounds[6 * (size_t)i + 3]);
-            if (c == 0 && s.objects[i].type != 0) { skip[i] = 1; continue; }
+            if (want_bounds) host_prim_bounds(s, c, i, &hb.bounds[6 * (size_t)i], &hb.bounds[6 * (size_t)i + 3]);
+            if (c == 0 && s.objects[i].type != 0) { hb.skip[i] = 1; continue; }
             // a superellipsoid flagged _pad0 = 1 is a CSG operand only (.sheen se_csg)
-            if (c == 4 && s.superellipsoids[i]._pad0 == 1) { skip[i] = 1; continue; }
+            if (c == 4 && s.superellipsoids[i]._pad0 == 1) { hb.skip[i] = 1; continue; }
             maxtr = std::fmax(maxtr, s.materials[prim_material(s, c, i)].transmission);
         }
         hb.opaque = maxtr <= 0.5f ? 1 : 0;
+        if (!build_bvh) continue;
         std::vector<float> mn(3 * (size_t)counts[c]), mx(3 * (size_t)counts[c]);
         for (int i = 0; i < counts[c]; i++)
             for (int a = 0; a < 3; a++) {
@@ -192,7 +195,7 @@ bool host_prepare_scene(const SheenScene& s, HostScene& out, std::string& err) {
             }
         WBvhHost wb;
         try {
-            wbvh_build(mn.data(), mx.data(), counts[c], skip.data(), WBVH_MAX_LEAF, 16, wb);
+            wbvh_build(mn.data(), mx.data(), counts[c], hb.skip.data(), WBVH_MAX_LEAF, 16, wb);
         } catch (const std::exception& e) {
             err = e.what();
             return false;

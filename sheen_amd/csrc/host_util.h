@@ -13,7 +13,8 @@
 struct HostBvh {
     std::vector<WNode> nodes;
     std::vector<uint32_t> prims;
-    std::vector<float> bounds;  // 6 per primitive (all primitives of the category)
+    std::vector<float> bounds;  // 6 per primitive (all primitives of the category); triangles: empty without a host build
+    std::vector<unsigned char> skip;  // primitives left out of the BVH (non-sphere objects, CSG-only superellipsoids)
     int count = 0;
     int opaque = 1;
     int max_stack = 0;
@@ -46,8 +47,10 @@ void host_prim_bounds(const SheenScene& s, int cat, int i, float* mn, float* mx)
 // 2x2 box mip chain with rounding (s+2)/4 (reference scene.h:230-261).
 void host_build_mips(HostTexture& t);
 
-// Validates references and builds every category BVH.  false + err on error.
-bool host_prepare_scene(const SheenScene& s, HostScene& out, std::string& err);
+// Validates references, computes primitive bounds and, with build_bvh, builds
+// every category BVH and the triangle leaf copies on the host (without it the
+// device builder does both).  false + err on error.
+bool host_prepare_scene(const SheenScene& s, HostScene& out, std::string& err, bool build_bvh = true);
 
 // Linear -> sRGB byte (reference png_color.h:11-21).
 unsigned char host_srgb_byte(float c);

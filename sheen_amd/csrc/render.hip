@@ -20,6 +20,7 @@
 #include "kernels.h"
 #include "sheen_c_api.h"
 #include "host_util.h"
+#include "wbvh_gpu.h"
 
 // ---------------------------------------------------------------------------
 // Error state (reference api/sheen_c_api.cu:21-26: static 512-byte buffer)
@@ -114,6 +115,8 @@ struct SheenContext {
         return ev_pool[ev_used++];
     }
     SheenRayStats last{};
+    bool bvh_on_device = true;
+    int bvh_nodes[N_BVH_CATS] = {}, bvh_prims[N_BVH_CATS] = {}, bvh_stack[N_BVH_CATS] = {}, bvh_depth[N_BVH_CATS] = {};
     Photon* photons = nullptr;  // sorted by cell
     int* cell_start = nullptr;
     int* cell_count = nullptr;
@@ -242,10 +245,78 @@ bool photon_pass(SheenContext* c, const SheenScene* in) {
     return ok;
 }
 
-// Uploads a scene: BVHs, records, textures.  Returns nullptr on error.
-static SheenContext* context_create(const SheenScene* in) {
+// Triangle leaf copies in leaf-slot order: v0 + index, v1, v2 (48 bytes).
+__global__ void k_tri_leaf(const uint32_t* prims, int n, const Triangle* tris, float4* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = prims[k];
+    const Triangle& t = tris[i];
+    out[3 * (size_t)k] = make_float4(t.v0.x, t.v0.y, t.v0.z, __int_as_float((int)i));
+    out[3 * (size_t)k + 1] = make_float4(t.v1.x, t.v1.y, t.v1.z, 0.0f);
+    out[3 * (size_t)k + 2] = make_float4(t.v2.x, t.v2.y, t.v2.z, 0.0f);
+}
+
+// Builds every category BVH on the device from the primitive bounds (the
+// host's for all kinds but triangles, whose bounds come from the uploaded
+// records).  Same trees, byte for byte, as the host build.
+static bool device_bvhs(SheenContext* c, const SheenScene* in, const HostScene& hs) {
+    DScene& d = c->ds;
+    for (int k = 0; k < N_BVH_CATS; k++) {
+        const HostBvh& hb = hs.bvh[k];
+        d.count[k] = 0;
+        d.cat_opaque[k] = hb.opaque;
+        if (hb.count == 0) continue;
+        float* box = nullptr;
+        bool own_box = false;
+        if (k == C_TRI) {
+            if (!alloc_dev((void**)&box, 6 * sizeof(float) * (size_t)hb.count)) { sheen_set_error("BVH bounds alloc failed"); return false; }
+            own_box = true;
+            wbvh_triangle_bounds(d.triangles, hb.count, box, c->stream);
+        } else {
+            box = c->upload(hb.bounds.data(), hb.bounds.size());
+            d.bounds[k] = box;
+            if (!box) { sheen_set_error("BVH bounds upload failed"); return false; }
+        }
+        unsigned char* skip = nullptr;
+        bool any_skip = false;
+        for (unsigned char f : hb.skip) any_skip = any_skip || f;
+        if (any_skip && !alloc_dev((void**)&skip, hb.skip.size())) { sheen_set_error("BVH flags alloc failed"); return false; }
+        if (skip) HIP_OR(hipMemcpy(skip, hb.skip.data(), hb.skip.size(), hipMemcpyHostToDevice), false);
+        WBvhDevice wd;
+        std::string err;
+        const bool ok = wbvh_build_device(box, hb.count, skip, WBVH_MAX_LEAF, c->stream, wd, err);
+        if (skip) (void)hipFree(skip);
+        if (own_box) (void)hipFree(box);
+        if (!ok) { sheen_set_error("%s", err.c_str()); return false; }
+        if (wd.n_nodes == 0) continue;
+        c->owned.push_back(wd.nodes);
+        c->owned.push_back(wd.prims);
+        if (wd.max_stack > 128) { sheen_set_error("BVH too deep for the traversal stack"); return false; }
+        d.bvh[k] = wd.nodes;
+        d.prim_ids[k] = wd.prims;
+        d.count[k] = hb.count;
+        c->bvh_nodes[k] = wd.n_nodes;
+        c->bvh_prims[k] = wd.n_prims;
+        c->bvh_stack[k] = wd.max_stack;
+        c->bvh_depth[k] = wd.depth;
+        if (k == C_TRI) {
+            float4* leaf = c->alloc_owned<float4>(3 * (size_t)wd.n_prims);
+            if (!leaf) { sheen_set_error("triangle leaf alloc failed"); return false; }
+            k_tri_leaf<<<(wd.n_prims + 255) / 256, 256, 0, c->stream>>>(wd.prims, wd.n_prims, (const Triangle*)d.triangles, leaf);
+            d.tri_leaf = leaf;
+            HIP_OR(hipGetLastError(), false);
+        }
+    }
+    HIP_OR(hipStreamSynchronize(c->stream), false);
+    return true;
+}
+
+// Uploads a scene: BVHs, records, textures.  flags: SHEEN_CTX_HOST_BVH builds
+// the BVHs on the host instead of the device.  Returns nullptr on error.
+static SheenContext* context_create(const SheenScene* in, int flags = 0) {
     if (!in) { sheen_set_error("null scene"); return nullptr; }
     if (in->num_materials <= 0 || !in->materials) { sheen_set_error("no materials"); return nullptr; }
+    const auto t0 = std::chrono::steady_clock::now();
     std::unique_ptr<SheenContext> c(new SheenContext());
     HIP_OR(hipGetDevice(&c->device), nullptr);
     hipDeviceProp_t prop;
@@ -257,31 +328,42 @@ static SheenContext* context_create(const SheenScene* in) {
     HIP_OR(hipEventCreateWithFlags(&c->ev_join[0], hipEventDisableTiming), nullptr);
     HIP_OR(hipEventCreateWithFlags(&c->ev_join[1], hipEventDisableTiming), nullptr);
 
+    const bool host_bvh = (flags & SHEEN_CTX_HOST_BVH) != 0;
+    c->bvh_on_device = !host_bvh;
     HostScene hs;
     std::string err;
-    if (!host_prepare_scene(*in, hs, err)) { sheen_set_error("%s", err.c_str()); return nullptr; }
+    if (!host_prepare_scene(*in, hs, err, host_bvh)) { sheen_set_error("%s", err.c_str()); return nullptr; }
     c->info = hs.info;
     DScene& d = c->ds;
     std::memset(&d, 0, sizeof(d));
-    for (int k = 0; k < N_BVH_CATS; k++) {
-        const HostBvh& hb = hs.bvh[k];
-        d.count[k] = hb.nodes.empty() ? 0 : hb.count;
-        d.cat_opaque[k] = hb.opaque;
-        if (hb.nodes.empty()) continue;
-        d.bvh[k] = c->upload(hb.nodes.data(), hb.nodes.size());
-        d.prim_ids[k] = c->upload(hb.prims.data(), hb.prims.size());
-        d.bounds[k] = c->upload(hb.bounds.data(), hb.bounds.size());
-        if (!d.bvh[k] || !d.prim_ids[k]) { sheen_set_error("BVH upload failed"); return nullptr; }
+    d.triangles = c->upload(in->triangles, in->triangles ? (size_t)in->num_triangles : 0);
+    d.num_triangles = in->triangles ? in->num_triangles : 0;
+    if (host_bvh) {
+        for (int k = 0; k < N_BVH_CATS; k++) {
+            const HostBvh& hb = hs.bvh[k];
+            d.count[k] = hb.nodes.empty() ? 0 : hb.count;
+            d.cat_opaque[k] = hb.opaque;
+            if (hb.nodes.empty()) continue;
+            d.bvh[k] = c->upload(hb.nodes.data(), hb.nodes.size());
+            d.prim_ids[k] = c->upload(hb.prims.data(), hb.prims.size());
+            d.bounds[k] = c->upload(hb.bounds.data(), hb.bounds.size());
+            if (!d.bvh[k] || !d.prim_ids[k]) { sheen_set_error("BVH upload failed"); return nullptr; }
+            c->bvh_nodes[k] = (int)hb.nodes.size();
+            c->bvh_prims[k] = (int)hb.prims.size();
+            c->bvh_stack[k] = hb.max_stack;
+            c->bvh_depth[k] = hb.depth;
+        }
+        if (!hs.tri_leaf.empty()) {
+            d.tri_leaf = c->upload((const float4*)hs.tri_leaf.data(), hs.tri_leaf.size() / 4);
+            if (!d.tri_leaf) { sheen_set_error("triangle upload failed"); return nullptr; }
+        }
+    } else if (!device_bvhs(c.get(), in, hs)) {
+        return nullptr;
     }
-    if (!hs.tri_leaf.empty()) {
-        d.tri_leaf = c->upload((const float4*)hs.tri_leaf.data(), hs.tri_leaf.size() / 4);
-        if (!d.tri_leaf) { sheen_set_error("triangle upload failed"); return nullptr; }
-    }
+    c->info.bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     d.objects = c->upload(in->objects, in->objects ? (size_t)in->num_objects : 0);
     d.strands = c->upload(in->sweep_strands, in->sweep_strands ? (size_t)in->num_sweep_strands : 0);
     d.sweep_points = c->upload(in->sweep_points, in->sweep_points ? (size_t)in->num_sweep_points : 0);
-    d.triangles = c->upload(in->triangles, in->triangles ? (size_t)in->num_triangles : 0);
-    d.num_triangles = in->triangles ? in->num_triangles : 0;
     // height fields: image pointers become device pointers (reference api/sheen_c_api.cu:220-236)
     if (in->heightfields && in->num_heightfields > 0) {
         std::vector<HeightField> hf(in->heightfields, in->heightfields + in->num_heightfields);
@@ -620,6 +702,31 @@ void sheen_cleanup(void) { (void)hipDeviceSynchronize(); }
 const char* sheen_get_last_error(void) { return g_error; }
 
 SheenContext* sheen_context_create(const SheenScene* scene) { return context_create(scene); }
+
+SheenContext* sheen_context_create_ex(const SheenScene* scene, int flags) { return context_create(scene, flags); }
+
+int sheen_context_bvh(const SheenContext* ctx, int category, void* nodes, int node_cap, unsigned int* prims, int prim_cap,
+                      int* info) {
+    if (!ctx || category < 0 || category >= N_BVH_CATS) { sheen_set_error("bad context or category"); return -1; }
+    const int nn = ctx->bvh_nodes[category], np = ctx->bvh_prims[category];
+    if (info) {
+        info[0] = nn;
+        info[1] = np;
+        info[2] = ctx->bvh_stack[category];
+        info[3] = ctx->bvh_depth[category];
+    }
+    if (nodes && nn > 0) {
+        if (node_cap < nn) { sheen_set_error("node buffer too small"); return -1; }
+        HIP_OR(hipMemcpy(nodes, ctx->ds.bvh[category], (size_t)nn * sizeof(WNode), hipMemcpyDeviceToHost), -1);
+    }
+    if (prims && np > 0) {
+        if (prim_cap < np) { sheen_set_error("primitive buffer too small"); return -1; }
+        HIP_OR(hipMemcpy(prims, ctx->ds.prim_ids[category], (size_t)np * sizeof(uint32_t), hipMemcpyDeviceToHost), -1);
+    }
+    return 0;
+}
+
+double sheen_context_build_ms(const SheenContext* ctx) { return ctx ? ctx->info.bvh_build_ms : -1.0; }
 
 void sheen_context_destroy(SheenContext* ctx) { delete ctx; }
 

@@ -6,6 +6,7 @@
 // Closest-hit results do not depend on the tree, only on which primitives a
 // ray can reach; the traversal keeps that set a superset of the reference's.
 #include "wbvh.h"
+#include "wbvh_rules.h"
 
 #include <algorithm>
 #include <cmath>
@@ -15,102 +16,74 @@
 
 namespace {
 
-struct Aabb {
-    float mn[3], mx[3];
-    void reset() { mn[0] = mn[1] = mn[2] = INFINITY; mx[0] = mx[1] = mx[2] = -INFINITY; }
-    void grow(const Aabb& b) {
-        for (int a = 0; a < 3; a++) { mn[a] = std::fmin(mn[a], b.mn[a]); mx[a] = std::fmax(mx[a], b.mx[a]); }
-    }
-    void grow_pt(const float* p) {
-        for (int a = 0; a < 3; a++) { mn[a] = std::fmin(mn[a], p[a]); mx[a] = std::fmax(mx[a], p[a]); }
-    }
-    double area() const {
-        double dx = (double)mx[0] - mn[0], dy = (double)mx[1] - mn[1], dz = (double)mx[2] - mn[2];
-        if (!(dx >= 0) || !(dy >= 0) || !(dz >= 0)) return 0.0;
-        return 2.0 * (dx * dy + dy * dz + dz * dx);
-    }
-};
-
 struct BinNode {
-    Aabb box;
+    WbBox box;
     int left = -1, right = -1;  // children (internal)
     int start = 0, count = 0;   // primitive range (leaf when count > 0)
 };
 
 struct Builder {
-    const float* bmin;
-    const float* bmax;
-    int max_leaf, bins;
+    int max_leaf;
     std::vector<int> idx;
     std::vector<float> cen;  // 3 per prim
-    std::vector<Aabb> pbox;
+    std::vector<WbBox> pbox;
     std::vector<BinNode> nodes;
 
-    Aabb range_box(int s, int e) const {
-        Aabb b; b.reset();
-        for (int i = s; i < e; i++) b.grow(pbox[idx[i]]);
+    WbBox range_box(int s, int e) const {
+        WbBox b;
+        wb_reset(b);
+        for (int i = s; i < e; i++) wb_grow(b, pbox[idx[i]]);
         return b;
     }
 
-    // Returns node index.  Iterative top-down split with an explicit work list.
+    // Iterative top-down binned-SAH split with an explicit work list.  The
+    // partition is stable, so primitives keep their input order within a
+    // node (the device builder reproduces it with prefix sums).
     void build(int n) {
         nodes.reserve(2 * (size_t)n / std::max(1, max_leaf) + 4);
         struct Work { int node, s, e; };
         std::vector<Work> work;
         nodes.push_back(BinNode());
         work.push_back({0, 0, n});
-        std::vector<Aabb> bb(bins), lb(bins);
-        std::vector<int> bc(bins), lc(bins);
+        WbBox bb[3][WBVH_BINS];
+        int bc[3][WBVH_BINS];
         while (!work.empty()) {
             Work w = work.back();
             work.pop_back();
             int s = w.s, e = w.e, cnt = e - s;
             nodes[w.node].box = range_box(s, e);
             if (cnt <= max_leaf) { nodes[w.node].start = s; nodes[w.node].count = cnt; continue; }
-            Aabb cb; cb.reset();
-            for (int i = s; i < e; i++) cb.grow_pt(&cen[3 * (size_t)idx[i]]);
-            int best_axis = -1, best_bin = -1;
-            double best_cost = INFINITY;
+            WbBox cb;
+            wb_reset(cb);
+            for (int i = s; i < e; i++) {
+                const float* c = &cen[3 * (size_t)idx[i]];
+                const WbBox pt = {{c[0], c[1], c[2]}, {c[0], c[1], c[2]}};
+                wb_grow(cb, pt);
+            }
             for (int a = 0; a < 3; a++) {
-                float lo = cb.mn[a], ext = cb.mx[a] - cb.mn[a];
+                for (int b = 0; b < WBVH_BINS; b++) { wb_reset(bb[a][b]); bc[a][b] = 0; }
+                const float ext = cb.mx[a] - cb.mn[a];
                 if (!(ext > 0.0f)) continue;
-                float k = (float)bins / ext;
-                for (int b = 0; b < bins; b++) { bb[b].reset(); bc[b] = 0; }
+                const float lo = cb.mn[a], k = wb_bin_scale(ext);
                 for (int i = s; i < e; i++) {
-                    int p = idx[i];
-                    int b = (int)((cen[3 * (size_t)p + a] - lo) * k);
-                    b = std::min(std::max(b, 0), bins - 1);
-                    bb[b].grow(pbox[p]);
-                    bc[b]++;
-                }
-                Aabb acc; acc.reset();
-                int ac = 0;
-                for (int b = 0; b < bins - 1; b++) {
-                    acc.grow(bb[b]); ac += bc[b];
-                    lb[b] = acc; lc[b] = ac;
-                }
-                acc.reset(); ac = 0;
-                for (int b = bins - 1; b >= 1; b--) {
-                    acc.grow(bb[b]); ac += bc[b];
-                    int l = lc[b - 1];
-                    if (l == 0 || ac == 0) continue;
-                    double c = lb[b - 1].area() * l + acc.area() * ac;
-                    if (c < best_cost) { best_cost = c; best_axis = a; best_bin = b; }
+                    const int p = idx[i];
+                    const int b = wb_bin_of(cen[3 * (size_t)p + a], lo, k);
+                    wb_grow(bb[a][b], pbox[p]);
+                    bc[a][b]++;
                 }
             }
+            int best_axis, best_bin;
+            wb_sah(cb.mn, cb.mx, [&](int a, int b) -> const WbBox& { return bb[a][b]; },
+                   [&](int a, int b) { return bc[a][b]; }, best_axis, best_bin);
             int mid;
             if (best_axis < 0) {
                 mid = s + cnt / 2;  // all centroids coincide: split by index
             } else {
-                float lo = cb.mn[best_axis], ext = cb.mx[best_axis] - cb.mn[best_axis];
-                float k = (float)bins / ext;
-                auto it = std::partition(idx.begin() + s, idx.begin() + e, [&](int p) {
-                    int b = (int)((cen[3 * (size_t)p + best_axis] - lo) * k);
-                    b = std::min(std::max(b, 0), bins - 1);
-                    return b < best_bin;
+                const float lo = cb.mn[best_axis], k = wb_bin_scale(cb.mx[best_axis] - cb.mn[best_axis]);
+                auto it = std::stable_partition(idx.begin() + s, idx.begin() + e, [&](int p) {
+                    return wb_bin_of(cen[3 * (size_t)p + best_axis], lo, k) < best_bin;
                 });
-                mid = (int)(it - idx.begin());
-                if (mid == s || mid == e) mid = s + cnt / 2;
+                mid = (int)(it - idx.begin());  // both sides non-empty by the SAH choice
             }
             int l = (int)nodes.size();
             nodes.push_back(BinNode());
@@ -124,58 +97,6 @@ struct Builder {
     }
 };
 
-// Exponent byte e (scale 2^(e-127)) such that extent/scale <= 65000.
-uint8_t pick_exp(float ext) {
-    if (!(ext > 0.0f)) return 1;
-    int k;
-    std::frexp((double)ext / 65000.0, &k);  // ext/65000 < 2^k
-    int e = k + 127;
-    if (e < 1) e = 1;
-    if (e > 254) e = 254;
-    return (uint8_t)e;
-}
-inline float exp_scale(uint8_t e) {
-    uint32_t bits = (uint32_t)e << 23;
-    float f;
-    std::memcpy(&f, &bits, 4);
-    return f;
-}
-
-void quantize(WNode& wn, const Aabb* cb, int nc) {
-    Aabb nb; nb.reset();
-    for (int i = 0; i < nc; i++) nb.grow(cb[i]);
-    float org[3] = {nb.mn[0], nb.mn[1], nb.mn[2]};
-    wn.ox = org[0]; wn.oy = org[1]; wn.oz = org[2];
-    uint8_t ex[3];
-    for (int a = 0; a < 3; a++) ex[a] = pick_exp(nb.mx[a] - nb.mn[a]);
-    wn.ex = ex[0]; wn.ey = ex[1]; wn.ez = ex[2];
-    for (int a = 0; a < 3; a++) {
-        for (;;) {
-            float sc = exp_scale(ex[a]);
-            bool ok = true;
-            for (int i = 0; i < WBVH_WIDTH; i++) {
-                if (i >= nc) { wn.q[2 * a][i] = 65535; wn.q[2 * a + 1][i] = 0; continue; }
-                double lo = std::floor(((double)cb[i].mn[a] - (double)org[a]) / (double)sc);
-                double hi = std::ceil(((double)cb[i].mx[a] - (double)org[a]) / (double)sc);
-                long ql = (long)std::max(0.0, lo), qh = (long)std::max(0.0, hi);
-                while (ql > 0 && std::fmaf((float)ql, sc, org[a]) > cb[i].mn[a]) ql--;
-                while (qh < 65535 && std::fmaf((float)qh, sc, org[a]) < cb[i].mx[a]) qh++;
-                if (std::fmaf((float)ql, sc, org[a]) > cb[i].mn[a] || std::fmaf((float)qh, sc, org[a]) < cb[i].mx[a] ||
-                    qh > 65535) {
-                    ok = false;
-                    break;
-                }
-                wn.q[2 * a][i] = (uint16_t)ql;
-                wn.q[2 * a + 1][i] = (uint16_t)qh;
-            }
-            if (ok) break;
-            if (ex[a] >= 254) throw std::runtime_error("wbvh: cannot quantise child box");
-            ex[a]++;
-        }
-    }
-    wn.ex = ex[0]; wn.ey = ex[1]; wn.ez = ex[2];
-}
-
 }  // namespace
 
 void wbvh_build(const float* bmin, const float* bmax, int n, const unsigned char* skip, int max_leaf, int bins,
@@ -187,16 +108,16 @@ void wbvh_build(const float* bmin, const float* bmax, int n, const unsigned char
     out.leaf_count = 0;
     if (max_leaf < 1) max_leaf = 1;
     if (max_leaf > 7) max_leaf = 7;
+    if (bins != WBVH_BINS) throw std::runtime_error("wbvh: the builder bins 16 ways");
     Builder B;
-    B.bmin = bmin; B.bmax = bmax; B.max_leaf = max_leaf; B.bins = bins < 2 ? 2 : bins;
+    B.max_leaf = max_leaf;
     B.pbox.resize(n);
     B.cen.resize(3 * (size_t)n);
     for (int i = 0; i < n; i++) {
         for (int a = 0; a < 3; a++) {
-            B.pbox[i].mn[a] = bmin[3 * (size_t)i + a];
-            B.pbox[i].mx[a] = bmax[3 * (size_t)i + a];
-            // centroid in double then rounded: robust for the +-1e30 bounds of lathes
-            B.cen[3 * (size_t)i + a] = (float)(0.5 * ((double)bmin[3 * (size_t)i + a] + (double)bmax[3 * (size_t)i + a]));
+            B.pbox[i].mn[a] = wb_canon(bmin[3 * (size_t)i + a]);
+            B.pbox[i].mx[a] = wb_canon(bmax[3 * (size_t)i + a]);
+            B.cen[3 * (size_t)i + a] = wb_centroid(B.pbox[i].mn[a], B.pbox[i].mx[a]);
         }
         if (!skip || !skip[i]) B.idx.push_back(i);
     }
@@ -216,24 +137,12 @@ void wbvh_build(const float* bmin, const float* bmax, int n, const unsigned char
     level.push_back(1);
     for (size_t qi = 0; qi < queue.size(); qi++) {
         Item it = queue[qi];
-        std::vector<int> ch;
-        const BinNode& bn = B.nodes[it.bin];
-        if (bn.count > 0) ch.push_back(it.bin);
-        else { ch.push_back(bn.left); ch.push_back(bn.right); }
-        while ((int)ch.size() < WBVH_WIDTH) {
-            int best = -1;
-            double ba = -1.0;
-            for (int k = 0; k < (int)ch.size(); k++) {
-                const BinNode& c = B.nodes[ch[k]];
-                if (c.count > 0) continue;
-                double a = c.box.area();
-                if (a > ba) { ba = a; best = k; }
-            }
-            if (best < 0) break;
-            int b = ch[best];
-            ch[best] = B.nodes[b].left;
-            ch.push_back(B.nodes[b].right);
-        }
+        int chv[WBVH_WIDTH];
+        const int nch = wb_collapse(it.bin, [&](int i, WbBox& box, int& l, int& r, int& cnt) {
+            const BinNode& b = B.nodes[i];
+            box = b.box; l = b.left; r = b.right; cnt = b.count;
+        }, chv);
+        std::vector<int> ch(chv, chv + nch);
         WNode wn;
         std::memset(&wn, 0, sizeof(wn));
         std::memset(wn.meta, 0xFF, sizeof(wn.meta));
@@ -242,7 +151,7 @@ void wbvh_build(const float* bmin, const float* bmax, int n, const unsigned char
         for (int c : ch) if (B.nodes[c].count == 0) ni++;
         wn.child_base = (uint32_t)out.nodes.size();
         wn.prim_base = (uint32_t)out.prims.size();
-        Aabb cb[WBVH_WIDTH];
+        WbBox cb[WBVH_WIDTH];
         int k_int = 0, off = 0;
         for (int k = 0; k < (int)ch.size(); k++) {
             const BinNode& c = B.nodes[ch[k]];
@@ -253,10 +162,7 @@ void wbvh_build(const float* bmin, const float* bmax, int n, const unsigned char
                 off += c.count;
                 out.leaf_count++;
             } else {
-                wn.meta[k] = (uint8_t)k_int;
-                int wi = (int)out.nodes.size() + 0;  // allocated below in order
-                (void)wi;
-                k_int++;
+                wn.meta[k] = (uint8_t)k_int++;  // internal children are allocated below, in slot order
             }
         }
         // allocate internal children contiguously in slot order
@@ -268,7 +174,7 @@ void wbvh_build(const float* bmin, const float* bmax, int n, const unsigned char
             level.push_back(level[it.wide] + 1);
             queue.push_back({ch[k], wi});
         }
-        quantize(wn, cb, (int)ch.size());
+        if (!wb_quantize(wn, cb, (int)ch.size())) throw std::runtime_error("wbvh: cannot quantise child box");
         out.nodes[it.wide] = wn;
         if ((int)n_internal.size() <= it.wide) n_internal.resize(it.wide + 1, 0);
         n_internal[it.wide] = ni;

@@ -58,6 +58,9 @@ def capi():
     _sig(lib, "sheen_render_scene", C.c_int, [S, C.c_int, C.c_int, C.c_int, C.c_char_p])
     _sig(lib, "sheen_get_last_error", C.c_char_p, [])
     _sig(lib, "sheen_context_create", P, [S])
+    _sig(lib, "sheen_context_create_ex", P, [S, C.c_int])
+    _sig(lib, "sheen_context_bvh", C.c_int, [P, C.c_int, P, C.c_int, P, C.c_int, C.POINTER(C.c_int)])
+    _sig(lib, "sheen_context_build_ms", C.c_double, [P])
     _sig(lib, "sheen_context_destroy", None, [P])
     _sig(lib, "sheen_context_render", C.c_int, [P, C.c_int, C.c_int, C.c_int, P, C.POINTER(SheenRayStats)])
     _sig(lib, "sheen_context_frame_device", P, [P])

@@ -45,14 +45,31 @@ def _scene_ptr(scene):
 
 
 class Context:
-    """A scene resident on the GPU (BVHs built once) for repeated renders."""
+    """A scene resident on the GPU (BVHs built once) for repeated renders.
+    The BVHs are built on the device; host_bvh=True builds the same trees
+    with the host builder."""
 
-    def __init__(self, scene):
+    def __init__(self, scene, host_bvh=False):
         lib = native.capi()
         self._lib = lib
-        self._c = lib.sheen_context_create(_scene_ptr(scene))
+        self._c = lib.sheen_context_create_ex(_scene_ptr(scene), 1 if host_bvh else 0)
         if not self._c:
             raise RuntimeError(native.last_error())
+
+    def bvh(self, category):
+        """(nodes (N, 128) uint8, prims (P,) uint32, max_stack, depth) of the
+        wide BVH of one primitive kind (0 spheres ... 6 prisms)."""
+        info = (C.c_int * 4)()
+        native.check(self._lib.sheen_context_bvh(self._c, category, None, 0, None, 0, info))
+        nodes = np.zeros((info[0], 128), np.uint8)
+        prims = np.zeros(info[1], np.uint32)
+        native.check(self._lib.sheen_context_bvh(self._c, category, native.as_ptr(nodes), info[0], native.as_ptr(prims),
+                                                 info[1], info))
+        return nodes, prims, info[2], info[3]
+
+    @property
+    def build_ms(self):
+        return self._lib.sheen_context_build_ms(self._c)
 
     def render(self, width, height, aa, download=True):
         st = native.SheenRayStats()
