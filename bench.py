@@ -8,10 +8,13 @@ glass, 2 point lights + 1 area light), traced through the full reference path
 is every traversal call of the timed frames divided by their wall time, with
 the scene already resident in HBM.
 
-Also measured (secondary, in the same JSON line): "batch", the 256-scene
-uneven batch as one pipeline against one render per scene (speedup), and
-"adjoint", forward and backward time of the differentiable-shading workload
-(adjoint / forward).
+Also measured (secondary, in the same JSON line): "scene_setup", the time
+to validate, build the BVHs of and upload the 1M-triangle scene with the
+device and with the host builder; "analytic", Mrays/s of the 10k
+analytic-primitive + CSG + caustics workload at 1280x720 x 16 spp; "batch",
+the 256-scene uneven batch as one pipeline against one render per scene
+(speedup); and "adjoint", forward and backward time of the
+differentiable-shading workload (adjoint / forward).
 
 Prints one JSON line (see --help).  --gpus above 1 is refused: the path runs
 on one GPU.
@@ -97,6 +100,37 @@ def adjoint_leg(steps):
             "parameters": {k: int(v.numel()) for k, v in p.items()}}
 
 
+def analytic_leg(steps):
+    """The analytic-primitive workload: 10,000 seeded spheres, cones,
+    superellipsoids, Bezier lathes, 12-point prisms and sweeps, one
+    1024x1024 fBm height field, CSG trees over half of the spheres and
+    superellipsoids, glass spheres casting caustics from a photon map;
+    1280x720 at 16 spp.  Mrays/s over the frame; the photon pass runs once
+    when the scene is created."""
+    from sheen_amd import scenes, sheen
+    sc = scenes.generate(3)
+    t = time.perf_counter()
+    ctx = sheen.Context(sc.scene)
+    setup_s = time.perf_counter() - t
+    W, H, A = sc.width, sc.height, sc.aa
+    ctx.render(W, H, A, download=False)
+    rays = 0
+    t = time.perf_counter()
+    for _ in range(steps):
+        _, st = ctx.render(W, H, A, download=False)
+        rays += st["total_rays"]
+    dt = time.perf_counter() - t
+    emitted, stored = ctx.photon_stats()
+    ctx.close()
+    c = sc.scene.contents
+    return {"workload": "analytic-10k-720p: %d analytic primitives, %d CSG nodes, %dx%d, %d spp, caustics"
+                        % (c.num_objects + c.num_superellipsoids + c.num_lathes + c.num_prisms + c.num_sweep_strands,
+                           c.num_csg_nodes, W, H, A),
+            "mrays_s": round(rays / dt / 1e6, 1), "ms_per_frame": round(dt / steps * 1e3, 2),
+            "rays_per_frame": rays // steps, "photons_emitted": emitted, "photons_stored": stored,
+            "scene_setup_s": round(setup_s, 3)}
+
+
 def batch_leg(steps):
     """The uneven batch: 256 scenes (triangle counts log-uniform in
     [1e3, 2e5], square images of 128-1024 pixels, 1-8 spp) rendered as one
@@ -132,6 +166,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-shading leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-scene batch leg")
+    ap.add_argument("--no-analytic", action="store_true", help="skip the analytic-primitive (10k + CSG + caustics) leg")
     ap.add_argument("--detail", type=int, default=0, help="geodesic frequency override (default 229)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -149,7 +184,12 @@ def main():
     sc = scenes.generate(2, **kw)
     W, H, A = sc.width, sc.height, sc.aa
     ntri = sc.scene.contents.num_triangles
+    # scene setup (validation, bounds, BVH build, upload): device build, then
+    # the host builder for comparison
+    with sheen.Context(sc.scene, host_bvh=True) as hctx:
+        host_build_ms = hctx.build_ms
     ctx = sheen.Context(sc.scene)
+    setup = {"device_bvh_ms": round(ctx.build_ms, 1), "host_bvh_ms": round(host_build_ms, 1)}
 
     for _ in range(args.warmup):
         ctx.render(W, H, A, download=False)
@@ -224,7 +264,10 @@ def main():
         "kernel_ms_per_step": {k: round(tot[k] / args.steps, 3) for k in ("primary_ms", "trace_ms", "shadow_ms",
                                                                         "shade_ms")},
     }
+    line["scene_setup"] = setup
     ctx.close()
+    if not args.no_analytic:
+        line["analytic"] = analytic_leg(max(2, min(args.steps, 5)))
     if not args.no_batch:
         line["batch"] = batch_leg(max(2, min(args.steps, 3)))
     if not args.no_adjoint:
