@@ -74,6 +74,7 @@ struct DScene {
     const Prism* prisms;
     const PrismPoint* prism_points;
     const CSGNode* csg;
+    const float* csg_box;  // 6 floats per CSG node (min xyz, max xyz): contains every span of its subtree
     int csg_root;
     int num_csg;
     const Plane* planes;

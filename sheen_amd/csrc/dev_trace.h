@@ -297,7 +297,32 @@ SHD void traverse(const DScene& S, const DRay& r, v3 inv, float& cat_t, int& cat
 
 // CSG tree evaluation (reference integrator/csg.h:113-186) with an explicit
 // post-order stack.  Returns the hit distance and the interval record.
+// Whether the ray's line can meet a CSG node's spans: a slab test against the
+// node box widened by 2^-8 (L1 distance from the ray origin to the box centre
+// + the half extents).  A sphere span needs a discriminant >= 0, which its
+// float arithmetic can report for lines up to about 2^-9.5 (|o - c| + r)
+// outside the sphere, and a superellipsoid span starts with its own box
+// test; the margin exceeds both, so a culled subtree would have returned no
+// span.  A NaN in the test (a line on a widened face) counts as a miss only
+// where the line lies on the margin itself.
+SHD bool csg_line_meets(const DRay& r, v3 inv, const float* b) {
+    const float hx = 0.5f * (b[3] - b[0]), hy = 0.5f * (b[4] - b[1]), hz = 0.5f * (b[5] - b[2]);
+    const float cx = b[0] + hx, cy = b[1] + hy, cz = b[2] + hz;
+    const float pad = 0.00390625f * (fabsf(r.o.x - cx) + fabsf(r.o.y - cy) + fabsf(r.o.z - cz) + hx + hy + hz) + 1e-6f;
+    if (!(pad < 3.0e38f)) return true;  // infinite or NaN box: no culling
+    const float ax = (b[0] - pad - r.o.x) * inv.x, bx = (b[3] + pad - r.o.x) * inv.x;
+    const float ay = (b[1] - pad - r.o.y) * inv.y, by = (b[4] + pad - r.o.y) * inv.y;
+    const float az = (b[2] - pad - r.o.z) * inv.z, bz = (b[5] + pad - r.o.z) * inv.z;
+    const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+    const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+    return !(tn > tf);
+}
+
 SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& best_iv, float& t_out) {
+    const v3 inv = mk3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    // a subtree whose box the line misses contributes no span
+    auto meets = [&](int node) { return !S.csg_box || csg_line_meets(r, inv, S.csg_box + 6 * (size_t)node); };
+    if (!meets(S.csg_root)) return false;
     struct Frame { int node; int state; int nA; CSpan A[8]; };
     Frame fr[CSG_MAX_DEPTH];
     CSpan ret[8];
@@ -330,7 +355,7 @@ SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& be
         }
         if (f.state == 0) {  // evaluate left
             f.state = 1;
-            if (n.left >= 0 && top + 1 < CSG_MAX_DEPTH) {
+            if (n.left >= 0 && top + 1 < CSG_MAX_DEPTH && meets(n.left)) {
                 fr[top + 1].node = n.left; fr[top + 1].state = 0; fr[top + 1].nA = 0;
                 top++;
             } else {
@@ -342,7 +367,7 @@ SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& be
             f.nA = nret;
             for (int i = 0; i < nret; i++) f.A[i] = ret[i];
             f.state = 2;
-            if (n.right >= 0 && top + 1 < CSG_MAX_DEPTH) {
+            if (n.right >= 0 && top + 1 < CSG_MAX_DEPTH && meets(n.right)) {
                 fr[top + 1].node = n.right; fr[top + 1].state = 0; fr[top + 1].nA = 0;
                 top++;
             } else {

@@ -80,3 +80,13 @@ extern "C" int sheen_host_bvh_check(const SheenScene* s, int cat, int* info) {
     info[4] = violations;
     return 0;
 }
+
+// CSG node boxes the render library uploads (6 floats per node).  Returns
+// the node count.
+extern "C" int sheen_host_csg_boxes(const SheenScene* s, float* out) {
+    if (!s) return -1;
+    std::vector<float> b;
+    host_csg_boxes(*s, b);
+    if (out && !b.empty()) std::memcpy(out, b.data(), b.size() * sizeof(float));
+    return (int)(b.size() / 6);
+}

@@ -87,6 +87,76 @@ void host_prim_bounds(const SheenScene& s, int cat, int i, float* mn, float* mx)
     }
 }
 
+void host_csg_boxes(const SheenScene& s, std::vector<float>& out) {
+    const int n = (s.csg_nodes && s.num_csg_nodes > 0) ? s.num_csg_nodes : 0;
+    out.assign(6 * (size_t)n, 0.0f);
+    const float inf = INFINITY;
+    auto set = [&](int i, const float* b) { std::copy(b, b + 6, out.begin() + 6 * (size_t)i); };
+    const float empty[6] = {inf, inf, inf, -inf, -inf, -inf};
+    const float all[6] = {-inf, -inf, -inf, inf, inf, inf};
+    std::vector<char> state(n, 0);  // 0 new, 1 in progress, 2 done
+    struct Item { int node; bool expanded; };
+    std::vector<Item> stack;
+    for (int r = 0; r < n; r++) {
+        if (state[r]) continue;
+        stack.push_back({r, false});
+        while (!stack.empty()) {
+            Item it = stack.back();
+            stack.pop_back();
+            const int i = it.node;
+            const CSGNode& nd = s.csg_nodes[i];
+            if (nd.leaf_kind >= 0) {
+                float b[6];
+                std::copy(empty, empty + 6, b);
+                if (nd.leaf_kind == 0 && nd.leaf_index >= 0 && nd.leaf_index < s.num_objects && s.objects) {
+                    const Object& o = s.objects[nd.leaf_index];
+                    const float c[3] = {o.param1.x, o.param1.y, o.param1.z}, rr = std::fabs(o.param2);
+                    for (int a = 0; a < 3; a++) { b[a] = c[a] - rr; b[3 + a] = c[a] + rr; }
+                } else if (nd.leaf_kind == 1 && nd.leaf_index >= 0 && nd.leaf_index < s.num_superellipsoids &&
+                           s.superellipsoids) {
+                    const Superellipsoid& e = s.superellipsoids[nd.leaf_index];
+                    const float c[3] = {e.center.x, e.center.y, e.center.z};
+                    const float h[3] = {std::fabs(e.a), std::fabs(e.b), std::fabs(e.c)};
+                    for (int a = 0; a < 3; a++) { b[a] = c[a] - h[a]; b[3 + a] = c[a] + h[a]; }
+                }
+                bool finite = true;
+                for (int a = 0; a < 6; a++) finite = finite && std::isfinite(b[a]);
+                set(i, (finite || nd.leaf_kind > 1 || b[0] == inf) ? b : all);
+                state[i] = 2;
+                continue;
+            }
+            const int ch[2] = {nd.left, nd.right};
+            if (!it.expanded) {
+                state[i] = 1;
+                stack.push_back({i, true});
+                for (int k = 0; k < 2; k++) {
+                    const int c = ch[k];
+                    if (c < 0 || c >= n) continue;
+                    if (state[c] == 1) { set(i, all); continue; }  // cycle: no culling below it
+                    if (state[c] == 0) stack.push_back({c, false});
+                }
+                continue;
+            }
+            float cb[2][6];
+            for (int k = 0; k < 2; k++) {
+                const int c = ch[k];
+                if (c < 0 || c >= n) std::copy(empty, empty + 6, cb[k]);
+                else if (state[c] == 2) std::copy(out.begin() + 6 * (size_t)c, out.begin() + 6 * (size_t)c + 6, cb[k]);
+                else std::copy(all, all + 6, cb[k]);  // on a cycle
+            }
+            float b[6];
+            for (int a = 0; a < 3; a++) {
+                if (nd.op == 0) { b[a] = std::fmin(cb[0][a], cb[1][a]); b[3 + a] = std::fmax(cb[0][3 + a], cb[1][3 + a]); }
+                else if (nd.op == 1) { b[a] = std::fmax(cb[0][a], cb[1][a]); b[3 + a] = std::fmin(cb[0][3 + a], cb[1][3 + a]); }
+                else if (nd.op == 2) { b[a] = cb[0][a]; b[3 + a] = cb[0][3 + a]; }
+                else { b[a] = inf; b[3 + a] = -inf; }  // unknown op: no spans
+            }
+            set(i, b);
+            state[i] = 2;
+        }
+    }
+}
+
 void host_build_mips(HostTexture& t) {
     t.mips = 1;
     if (t.w < 2 && t.h < 2) return;

@@ -44,6 +44,13 @@ struct HostScene {
 // (bvh.h:24-66, 235-242, 304-331, 394-407).
 void host_prim_bounds(const SheenScene& s, int cat, int i, float* mn, float* mx);
 
+// Box of every CSG node (6 floats: min xyz, max xyz) containing every span
+// its subtree can return: a sphere or superellipsoid leaf's own box, the
+// union of the children's boxes for a union, their intersection for an
+// intersection, the left child's box for a difference.  Missing children
+// give an empty box; cycles and non-finite values an unbounded one.
+void host_csg_boxes(const SheenScene& s, std::vector<float>& out);
+
 // 2x2 box mip chain with rounding (s+2)/4 (reference scene.h:230-261).
 void host_build_mips(HostTexture& t);
 

@@ -383,6 +383,11 @@ static SheenContext* context_create(const SheenScene* in, int flags = 0) {
     d.prisms = c->upload(in->prisms, in->prisms ? (size_t)in->num_prisms : 0);
     d.prism_points = c->upload(in->prism_points, in->prism_points ? (size_t)in->num_prism_points : 0);
     d.csg = c->upload(in->csg_nodes, in->csg_nodes ? (size_t)in->num_csg_nodes : 0);
+    if (d.csg) {
+        std::vector<float> cbox;
+        host_csg_boxes(*in, cbox);
+        d.csg_box = c->upload(cbox.data(), cbox.size());
+    }
     d.num_csg = in->csg_nodes ? in->num_csg_nodes : 0;
     d.csg_root = (d.csg && in->csg_root >= 0 && in->csg_root < d.num_csg) ? in->csg_root : -1;
     d.planes = c->upload(in->planes, in->planes ? (size_t)in->num_planes : 0);
