@@ -297,32 +297,45 @@ SHD void traverse(const DScene& S, const DRay& r, v3 inv, float& cat_t, int& cat
 
 // CSG tree evaluation (reference integrator/csg.h:113-186) with an explicit
 // post-order stack.  Returns the hit distance and the interval record.
-// Whether the ray's line can meet a CSG node's spans: a slab test against the
-// node box widened by 2^-8 (L1 distance from the ray origin to the box centre
-// + the half extents).  A sphere span needs a discriminant >= 0, which its
-// float arithmetic can report for lines up to about 2^-9.5 (|o - c| + r)
-// outside the sphere, and a superellipsoid span starts with its own box
-// test; the margin exceeds both, so a culled subtree would have returned no
-// span.  A NaN in the test (a line on a widened face) counts as a miss only
-// where the line lies on the margin itself.
-SHD bool csg_line_meets(const DRay& r, v3 inv, const float* b) {
+// Entry and exit distances of the ray's line through a CSG node's box
+// widened by 2^-8 (L1 distance from the ray origin to the box centre + the
+// half extents).  Every span the node's subtree can return lies inside
+// [tn, tf]: a sphere span needs a discriminant >= 0, which the float
+// arithmetic can report for lines up to about 2^-9.5 (|o - c| + r) outside
+// the sphere and then places both chord ends within about 2^-10.5 |o - c| of
+// the closest approach; a superellipsoid span starts with its own box test;
+// intersections and differences return parts of their operands' spans.
+// Returns false for a box the line misses; a NaN in the test (a line on a
+// widened face) counts as a miss only where the line lies on the margin.
+SHD bool csg_line_box(const DRay& r, v3 inv, const float* b, float& tn, float& tf) {
     const float hx = 0.5f * (b[3] - b[0]), hy = 0.5f * (b[4] - b[1]), hz = 0.5f * (b[5] - b[2]);
     const float cx = b[0] + hx, cy = b[1] + hy, cz = b[2] + hz;
     const float pad = 0.00390625f * (fabsf(r.o.x - cx) + fabsf(r.o.y - cy) + fabsf(r.o.z - cz) + hx + hy + hz) + 1e-6f;
-    if (!(pad < 3.0e38f)) return true;  // infinite or NaN box: no culling
+    if (!(pad < 3.0e38f)) {  // unbounded or NaN box: no culling
+        tn = -__builtin_huge_valf();
+        tf = __builtin_huge_valf();
+        return true;
+    }
     const float ax = (b[0] - pad - r.o.x) * inv.x, bx = (b[3] + pad - r.o.x) * inv.x;
     const float ay = (b[1] - pad - r.o.y) * inv.y, by = (b[4] + pad - r.o.y) * inv.y;
     const float az = (b[2] - pad - r.o.z) * inv.z, bz = (b[5] + pad - r.o.z) * inv.z;
-    const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
-    const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+    tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+    tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
     return !(tn > tf);
 }
 
+// Subtrees whose widened box the line misses, or meets only outside
+// [t0, t1], are skipped: each leaf span is clipped to [t0, t1] and would be
+// empty, and so would every combination of them.
 SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& best_iv, float& t_out) {
     const v3 inv = mk3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
-    // a subtree whose box the line misses contributes no span
-    auto meets = [&](int node) { return !S.csg_box || csg_line_meets(r, inv, S.csg_box + 6 * (size_t)node); };
-    if (!meets(S.csg_root)) return false;
+    auto visit = [&](int node) {
+        if (!S.csg_box) return true;
+        float tn, tf;
+        if (!csg_line_box(r, inv, S.csg_box + 6 * (size_t)node, tn, tf)) return false;
+        return !(tn > t1 || tf < t0);
+    };
+    if (!visit(S.csg_root)) return false;
     struct Frame { int node; int state; int nA; CSpan A[8]; };
     Frame fr[CSG_MAX_DEPTH];
     CSpan ret[8];
@@ -355,7 +368,7 @@ SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& be
         }
         if (f.state == 0) {  // evaluate left
             f.state = 1;
-            if (n.left >= 0 && top + 1 < CSG_MAX_DEPTH && meets(n.left)) {
+            if (n.left >= 0 && top + 1 < CSG_MAX_DEPTH && visit(n.left)) {
                 fr[top + 1].node = n.left; fr[top + 1].state = 0; fr[top + 1].nA = 0;
                 top++;
             } else {
@@ -367,7 +380,7 @@ SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& be
             f.nA = nret;
             for (int i = 0; i < nret; i++) f.A[i] = ret[i];
             f.state = 2;
-            if (n.right >= 0 && top + 1 < CSG_MAX_DEPTH && meets(n.right)) {
+            if (n.right >= 0 && top + 1 < CSG_MAX_DEPTH && visit(n.right)) {
                 fr[top + 1].node = n.right; fr[top + 1].state = 0; fr[top + 1].nA = 0;
                 top++;
             } else {
