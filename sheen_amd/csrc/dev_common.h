@@ -53,6 +53,17 @@ struct PhotonMapView {
     float cell;
 };
 
+// Hierarchy over the subtrees hanging off the CSG root's union spine (the
+// maximal set of union nodes reachable from the root through unions).
+// Internal node: children a, b (>= 0); leaf: a = -(first + 1) into the
+// subtree list, b = count.  Boxes as the CSG node boxes.
+struct CsgSpineNode {
+    float lo[3];
+    int a;
+    float hi[3];
+    int b;
+};
+
 // Device scene: everything a kernel reads, passed by value.
 struct DScene {
     const WNode* bvh[N_BVH_CATS];
@@ -77,6 +88,14 @@ struct DScene {
     const float* csg_box;  // 6 floats per CSG node (min xyz, max xyz): contains every span of its subtree
     int csg_root;
     int num_csg;
+    // union spine of the CSG root (csg_spine != 0): subtrees as {node, in-order
+    // rank, frame depth, 0}; the first spine_always of them have unbounded
+    // boxes and are always evaluated, the rest are reached through spine_nodes
+    int csg_spine;
+    const CsgSpineNode* spine_nodes;
+    const int4* spine_sub;
+    int spine_n_nodes;
+    int spine_always;
     const Plane* planes;
     int num_planes;
     const Material* materials;

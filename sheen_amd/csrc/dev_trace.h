@@ -324,24 +324,25 @@ SHD bool csg_line_box(const DRay& r, v3 inv, const float* b, float& tn, float& t
     return !(tn > tf);
 }
 
-// Subtrees whose widened box the line misses, or meets only outside
-// [t0, t1], are skipped: each leaf span is clipped to [t0, t1] and would be
-// empty, and so would every combination of them.
-SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& best_iv, float& t_out) {
-    const v3 inv = mk3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+// Spans of the CSG subtree rooted at `root`, evaluated with `frames`
+// frames of the post-order stack (the root's frame included; a child that
+// would need more is not evaluated and contributes nothing).  Subtrees whose
+// widened box the line misses, or meets only outside [t0, t1], are skipped:
+// each leaf span is clipped to [t0, t1] and would be empty, and so would
+// every combination of them.
+SHD void csg_eval(const DScene& S, const DRay& r, v3 inv, float t0, float t1, int root, int frames, CSpan* ret, int& nret) {
     auto visit = [&](int node) {
         if (!S.csg_box) return true;
         float tn, tf;
         if (!csg_line_box(r, inv, S.csg_box + 6 * (size_t)node, tn, tf)) return false;
         return !(tn > t1 || tf < t0);
     };
-    if (!visit(S.csg_root)) return false;
+    nret = 0;
+    if (frames < 1 || !visit(root)) return;
     struct Frame { int node; int state; int nA; CSpan A[8]; };
     Frame fr[CSG_MAX_DEPTH];
-    CSpan ret[8];
-    int nret = 0;
     int top = 0;
-    fr[0].node = S.csg_root; fr[0].state = 0; fr[0].nA = 0;
+    fr[0].node = root; fr[0].state = 0; fr[0].nA = 0;
     while (top >= 0) {
         Frame& f = fr[top];
         const CSGNode& n = S.csg[f.node];
@@ -368,7 +369,7 @@ SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& be
         }
         if (f.state == 0) {  // evaluate left
             f.state = 1;
-            if (n.left >= 0 && top + 1 < CSG_MAX_DEPTH && visit(n.left)) {
+            if (n.left >= 0 && top + 1 < frames && visit(n.left)) {
                 fr[top + 1].node = n.left; fr[top + 1].state = 0; fr[top + 1].nA = 0;
                 top++;
             } else {
@@ -380,7 +381,7 @@ SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& be
             f.nA = nret;
             for (int i = 0; i < nret; i++) f.A[i] = ret[i];
             f.state = 2;
-            if (n.right >= 0 && top + 1 < CSG_MAX_DEPTH && visit(n.right)) {
+            if (n.right >= 0 && top + 1 < frames && visit(n.right)) {
                 fr[top + 1].node = n.right; fr[top + 1].state = 0; fr[top + 1].nA = 0;
                 top++;
             } else {
@@ -395,14 +396,107 @@ SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& be
         csg_combine(f.A, f.nA, B, nB, n.op, ret, nret);
         top--;
     }
-    float best = 1e30f;
-    int bi = -1;
-    for (int i = 0; i < nret; i++)
-        if (ret[i].te < best && ret[i].tx > t0) { best = ret[i].te; bi = i; }
-    if (bi < 0) return false;
-    float t = fmaxf(ret[bi].te, t0);
+}
+
+// CSG hit of a ray (reference integrator/csg.h:113-186): the first span, by
+// entry, whose exit lies beyond t0; the caller uses its entry (t) and its
+// material, leaf and kind.
+//
+// When the root is a union, the unions reachable from it through unions
+// only merge the spans of the subtrees below them: sorted by entry (stable,
+// so equal entries keep the in-order position of their subtree, then their
+// position in its list), overlapping spans fused into groups headed by their
+// first span, at most 8 groups kept (the farthest dropped).  Every span
+// starts at or after t0, so the only spans whose exit does not pass t0 are
+// points at t0, and the chosen group's head is the first span, by (entry,
+// subtree rank, position), whose entry equals the smallest entry E of the
+// spans that end beyond t0 — a point span at t0 counts when E = t0.  The
+// subtrees are therefore visited near-first through the spine hierarchy and
+// one whose box entry lies beyond the best E found so far is skipped.
+SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& best_iv, float& t_out) {
+    const v3 inv = mk3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    CSpan ret[8];
+    int nret = 0;
+    if (!S.csg_spine) {
+        csg_eval(S, r, inv, t0, t1, S.csg_root, CSG_MAX_DEPTH, ret, nret);
+        float best = 1e30f;
+        int bi = -1;
+        for (int i = 0; i < nret; i++)
+            if (ret[i].te < best && ret[i].tx > t0) { best = ret[i].te; bi = i; }
+        if (bi < 0) return false;
+        float t = fmaxf(ret[bi].te, t0);
+        if (t < t0 || t > t1) return false;
+        best_iv = ret[bi];
+        t_out = t;
+        return true;
+    }
+    // best span ending beyond t0 (E, rank, pos) and best point span at t0
+    float E = 1e30f;
+    int er = 0x7FFFFFFF, ep = 0, dr = 0x7FFFFFFF, dp = 0;
+    CSpan e_iv, d_iv;
+    e_iv.te = e_iv.tx = 0.0f; e_iv.mat = e_iv.leaf = e_iv.kind = 0;
+    d_iv = e_iv;
+    auto take = [&](int4 sub) {
+        csg_eval(S, r, inv, t0, t1, sub.x, CSG_MAX_DEPTH - sub.z, ret, nret);
+        for (int p = 0; p < nret; p++) {
+            const CSpan& sp = ret[p];
+            if (!(sp.te < 1e30f)) continue;
+            if (sp.tx > t0) {
+                if (sp.te < E || (sp.te == E && (sub.y < er || (sub.y == er && p < ep)))) { E = sp.te; er = sub.y; ep = p; e_iv = sp; }
+            } else if (sp.te == t0 && (sub.y < dr || (sub.y == dr && p < dp))) {
+                dr = sub.y; dp = p; d_iv = sp;
+            }
+        }
+    };
+    for (int k = 0; k < S.spine_always; k++) take(S.spine_sub[k]);
+    if (S.spine_n_nodes > 0) {
+        int stk[32];
+        float stn[32];
+        int sp = 0;
+        const CsgSpineNode* N = S.spine_nodes;
+        auto node_box = [&](int i, float& tn, float& tf) {
+            const CsgSpineNode& cn = N[i];
+            const float bx[6] = {cn.lo[0], cn.lo[1], cn.lo[2], cn.hi[0], cn.hi[1], cn.hi[2]};
+            return csg_line_box(r, inv, bx, tn, tf) && !(tn > t1 || tf < t0);
+        };
+        float tn0, tf0;
+        if (node_box(0, tn0, tf0)) { stk[0] = 0; stn[0] = tn0; sp = 1; }
+        // the hierarchy is balanced (median splits): at most log2(subtrees) + 1
+        // entries are ever stacked, far below 32
+        while (sp > 0) {
+            sp--;
+            const int i = stk[sp];
+            if (stn[sp] > E) continue;
+            const CsgSpineNode nd = N[i];
+            if (nd.a < 0) {
+                for (int k = 0; k < nd.b; k++) take(S.spine_sub[-nd.a - 1 + k]);
+                continue;
+            }
+            float tnc[2];
+            bool hit[2];
+            const int ch[2] = {nd.a, nd.b};
+            for (int c = 0; c < 2; c++) {
+                float a0, a1;
+                hit[c] = node_box(ch[c], a0, a1) && !(a0 > E);
+                tnc[c] = a0;
+            }
+            // nearer child popped first
+            const int first = (hit[0] && hit[1] && tnc[1] < tnc[0]) ? 1 : 0;
+            for (int q = 1; q >= 0; q--) {
+                const int c = q == 0 ? first : 1 - first;
+                if (!hit[c] || sp >= 32) continue;
+                stk[sp] = ch[c];
+                stn[sp] = tnc[c];
+                sp++;
+            }
+        }
+    }
+    if (!(E < 1e30f)) return false;
+    CSpan head = e_iv;
+    if (E == t0 && (dr < er || (dr == er && dp < ep))) head = d_iv;
+    const float t = fmaxf(head.te, t0);
     if (t < t0 || t > t1) return false;
-    best_iv = ret[bi];
+    best_iv = head;
     t_out = t;
     return true;
 }

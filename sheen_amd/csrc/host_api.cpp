@@ -90,3 +90,22 @@ extern "C" int sheen_host_csg_boxes(const SheenScene* s, float* out) {
     if (out && !b.empty()) std::memcpy(out, b.data(), b.size() * sizeof(float));
     return (int)(b.size() / 6);
 }
+
+// The CSG root's union spine as the render library builds it: info[0] = 1
+// when the root is a union, info[1] = subtrees, info[2] = leading unbounded
+// ones, info[3] = hierarchy nodes.  sub (4 ints per subtree: node, rank,
+// depth, 0) and nodes (8 words per node) may be null to query the sizes.
+extern "C" int sheen_host_csg_spine(const SheenScene* s, int* info, int* sub, void* nodes) {
+    if (!s || !info) return -1;
+    std::vector<float> b;
+    host_csg_boxes(*s, b);
+    HostCsgSpine sp;
+    host_csg_spine(*s, b, sp);
+    info[0] = sp.spine ? 1 : 0;
+    info[1] = (int)(sp.sub.size() / 4);
+    info[2] = sp.always;
+    info[3] = (int)sp.nodes.size();
+    if (sub && !sp.sub.empty()) std::memcpy(sub, sp.sub.data(), sp.sub.size() * sizeof(int));
+    if (nodes && !sp.nodes.empty()) std::memcpy(nodes, sp.nodes.data(), sp.nodes.size() * sizeof(HostSpineNode));
+    return 0;
+}

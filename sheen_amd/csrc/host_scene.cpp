@@ -147,13 +147,96 @@ void host_csg_boxes(const SheenScene& s, std::vector<float>& out) {
             float b[6];
             for (int a = 0; a < 3; a++) {
                 if (nd.op == 0) { b[a] = std::fmin(cb[0][a], cb[1][a]); b[3 + a] = std::fmax(cb[0][3 + a], cb[1][3 + a]); }
-                else if (nd.op == 1) { b[a] = std::fmax(cb[0][a], cb[1][a]); b[3 + a] = std::fmin(cb[0][3 + a], cb[1][3 + a]); }
-                else if (nd.op == 2) { b[a] = cb[0][a]; b[3 + a] = cb[0][3 + a]; }
+                else if (nd.op == 1 || nd.op == 2) { b[a] = cb[0][a]; b[3 + a] = cb[0][3 + a]; }
                 else { b[a] = inf; b[3 + a] = -inf; }  // unknown op: no spans
             }
             set(i, b);
             state[i] = 2;
         }
+    }
+}
+
+void host_csg_spine(const SheenScene& s, const std::vector<float>& boxes, HostCsgSpine& out) {
+    out = HostCsgSpine();
+    const int n = (s.csg_nodes && s.num_csg_nodes > 0) ? s.num_csg_nodes : 0;
+    const int root = s.csg_root;
+    if (root < 0 || root >= n) return;
+    auto spine_union = [&](int i) { return s.csg_nodes[i].leaf_kind < 0 && s.csg_nodes[i].op == 0; };
+    if (!spine_union(root)) return;
+    out.spine = true;
+    // in-order walk of the union spine; a node at frame depth >= CSG_DEPTH is
+    // never evaluated (the evaluation stack holds CSG_DEPTH frames)
+    struct Sub { int node, rank, depth; };
+    std::vector<Sub> subs;
+    std::vector<std::pair<int, int>> stack;  // (node, depth); right pushed first
+    stack.push_back({root, 0});
+    int rank = 0;
+    while (!stack.empty()) {
+        const auto [i, d] = stack.back();
+        stack.pop_back();
+        if (i < 0 || i >= n || d >= HOST_CSG_DEPTH) continue;
+        if (spine_union(i)) {
+            stack.push_back({s.csg_nodes[i].right, d + 1});
+            stack.push_back({s.csg_nodes[i].left, d + 1});
+            continue;
+        }
+        subs.push_back({i, rank++, d});
+    }
+    // unbounded boxes first (always evaluated), empty boxes dropped (no spans)
+    std::vector<Sub> bounded;
+    for (const Sub& u : subs) {
+        const float* b = &boxes[6 * (size_t)u.node];
+        bool finite = true, empty = false;
+        for (int a = 0; a < 3; a++) {
+            finite = finite && std::isfinite(b[a]) && std::isfinite(b[3 + a]);
+            empty = empty || b[a] > b[3 + a];
+        }
+        if (finite && empty) continue;
+        if (!finite) { out.sub.insert(out.sub.end(), {u.node, u.rank, u.depth, 0}); out.always++; }
+        else bounded.push_back(u);
+    }
+    if (bounded.empty()) return;
+    // median-split hierarchy, at most 2 subtrees per leaf
+    struct Range { int node, s, e; };
+    std::vector<Range> work;
+    out.nodes.push_back(HostSpineNode());
+    work.push_back({0, 0, (int)bounded.size()});
+    auto cen = [&](const Sub& u, int a) {
+        const float* b = &boxes[6 * (size_t)u.node];
+        return 0.5 * ((double)b[a] + (double)b[3 + a]);
+    };
+    while (!work.empty()) {
+        const Range w = work.back();
+        work.pop_back();
+        HostSpineNode nd;
+        for (int a = 0; a < 3; a++) { nd.lo[a] = INFINITY; nd.hi[a] = -INFINITY; }
+        for (int k = w.s; k < w.e; k++) {
+            const float* b = &boxes[6 * (size_t)bounded[k].node];
+            for (int a = 0; a < 3; a++) { nd.lo[a] = std::fmin(nd.lo[a], b[a]); nd.hi[a] = std::fmax(nd.hi[a], b[3 + a]); }
+        }
+        if (w.e - w.s <= 2) {
+            nd.a = -((int)(out.sub.size() / 4) + 1);
+            nd.b = w.e - w.s;
+            for (int k = w.s; k < w.e; k++)
+                out.sub.insert(out.sub.end(), {bounded[k].node, bounded[k].rank, bounded[k].depth, 0});
+        } else {
+            int axis = 0;
+            for (int a = 1; a < 3; a++)
+                if (nd.hi[a] - nd.lo[a] > nd.hi[axis] - nd.lo[axis]) axis = a;
+            const int mid = (w.s + w.e) / 2;
+            std::nth_element(bounded.begin() + w.s, bounded.begin() + mid, bounded.begin() + w.e,
+                             [&](const Sub& x, const Sub& y) {
+                                 const double cx = cen(x, axis), cy = cen(y, axis);
+                                 return cx < cy || (cx == cy && x.rank < y.rank);
+                             });
+            nd.a = (int)out.nodes.size();
+            out.nodes.push_back(HostSpineNode());
+            nd.b = (int)out.nodes.size();
+            out.nodes.push_back(HostSpineNode());
+            work.push_back({nd.a, w.s, mid});
+            work.push_back({nd.b, mid, w.e});
+        }
+        out.nodes[w.node] = nd;
     }
 }
 

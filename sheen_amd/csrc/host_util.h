@@ -46,10 +46,33 @@ void host_prim_bounds(const SheenScene& s, int cat, int i, float* mn, float* mx)
 
 // Box of every CSG node (6 floats: min xyz, max xyz) containing every span
 // its subtree can return: a sphere or superellipsoid leaf's own box, the
-// union of the children's boxes for a union, their intersection for an
-// intersection, the left child's box for a difference.  Missing children
-// give an empty box; cycles and non-finite values an unbounded one.
+// union of the children's boxes for a union, the left operand's box for an
+// intersection or a difference (their spans are parts of the left operand's).
+// Missing children give an empty box; cycles and non-finite values an
+// unbounded one.
 void host_csg_boxes(const SheenScene& s, std::vector<float>& out);
+
+// Frames of the device's CSG evaluation stack.
+#define HOST_CSG_DEPTH 16
+
+// The CSG root's union spine: the subtrees hanging off the unions reachable
+// from the root through unions, in the order an in-order evaluation meets
+// them (rank), with the frame depth of each, and a median-split hierarchy
+// over their boxes.  Layout of a node as the device's CsgSpineNode.
+struct HostSpineNode {
+    float lo[3];
+    int a;  // internal: first child; leaf: -(first subtree + 1)
+    float hi[3];
+    int b;  // internal: second child; leaf: subtree count
+};
+static_assert(sizeof(HostSpineNode) == 32, "spine node layout");
+struct HostCsgSpine {
+    bool spine = false;               // the root is a union
+    std::vector<HostSpineNode> nodes;
+    std::vector<int> sub;             // 4 ints per subtree: node, rank, depth, 0
+    int always = 0;                   // leading subtrees with unbounded boxes
+};
+void host_csg_spine(const SheenScene& s, const std::vector<float>& boxes, HostCsgSpine& out);
 
 // 2x2 box mip chain with rounding (s+2)/4 (reference scene.h:230-261).
 void host_build_mips(HostTexture& t);

@@ -245,6 +245,9 @@ bool photon_pass(SheenContext* c, const SheenScene* in) {
     return ok;
 }
 
+static_assert(HOST_CSG_DEPTH == CSG_MAX_DEPTH, "host and device CSG stack depths");
+static_assert(sizeof(HostSpineNode) == sizeof(CsgSpineNode), "spine node layout");
+
 // Triangle leaf copies in leaf-slot order: v0 + index, v1, v2 (48 bytes).
 __global__ void k_tri_leaf(const uint32_t* prims, int n, const Triangle* tris, float4* out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -387,6 +390,13 @@ static SheenContext* context_create(const SheenScene* in, int flags = 0) {
         std::vector<float> cbox;
         host_csg_boxes(*in, cbox);
         d.csg_box = c->upload(cbox.data(), cbox.size());
+        HostCsgSpine sp;
+        host_csg_spine(*in, cbox, sp);
+        d.csg_spine = sp.spine ? 1 : 0;
+        d.spine_nodes = c->upload(reinterpret_cast<const CsgSpineNode*>(sp.nodes.data()), sp.nodes.size());
+        d.spine_sub = c->upload(reinterpret_cast<const int4*>(sp.sub.data()), sp.sub.size() / 4);
+        d.spine_n_nodes = (int)sp.nodes.size();
+        d.spine_always = sp.always;
     }
     d.num_csg = in->csg_nodes ? in->num_csg_nodes : 0;
     d.csg_root = (d.csg && in->csg_root >= 0 && in->csg_root < d.num_csg) ? in->csg_root : -1;

@@ -109,6 +109,7 @@ def host():
     _sig(lib, "sheen_scene_free", None, [P])
     _sig(lib, "sheen_host_bvh_check", C.c_int, [C.POINTER(contract.SheenScene), C.c_int, C.POINTER(C.c_int)])
     _sig(lib, "sheen_host_csg_boxes", C.c_int, [C.POINTER(contract.SheenScene), C.c_void_p])
+    _sig(lib, "sheen_host_csg_spine", C.c_int, [C.POINTER(contract.SheenScene), C.POINTER(C.c_int), C.c_void_p, C.c_void_p])
     _sig(lib, "sheen_file_load", P, [C.c_char_p])
     _sig(lib, "sheen_file_error", C.c_char_p, [P])
     _sig(lib, "sheen_file_scene", C.POINTER(contract.SheenScene), [P])
