@@ -114,6 +114,7 @@ struct DScene {
     float caustics;
     int any_strands;
     int any_beams;
+    int wave_one_scene;  // 1: every lane of a kernel traces this scene (a single context); 0 in batch tables
     CameraData cam;
 };
 

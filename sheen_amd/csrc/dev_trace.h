@@ -56,7 +56,18 @@ struct TStack {
     uint2* lds;    // &shared[tid], stride TRACE_BLOCK
     uint2* spill;  // private array of SPILL_DEPTH entries
     uint32_t spills;
+    uint32_t* list;            // this wave's LEAF_LIST entries of pending triangle tests
+    unsigned long long* keys;  // this wave's 64 best-hit keys, one per lane
 };
+
+// Triangle tests of one node visit shared across the wave (see trav_step):
+// entries of the pending-test list per wave, and the slot bits of an entry
+// (owner lane above them).
+#ifndef LEAF_BATCH
+#define LEAF_BATCH 1
+#endif
+#define LEAF_LIST 256
+#define LEAF_SLOT_BITS 26
 SHD void stack_push(TStack& st, int& sp, uint2 e) {
     const uint32_t slot = (uint32_t)sp % LDS_DEPTH;
     if (sp >= LDS_DEPTH) {
@@ -227,6 +238,72 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
             const bool leaf = (m >> 5) != 0;
             keys[i] = (hit && !leaf) ? ((__float_as_uint(tn) & ~7u) | (m & 7u)) : 0xFFFFFFFFu;
             if (hit && leaf) leafmask |= 1u << i;
+        }
+    }
+    // Triangle leaves: when some lane has two or more triangles to test, the
+    // wave's pending tests are listed in LDS and dealt to its active lanes,
+    // each testing another lane's ray (fetched by cross-lane permute) and
+    // merging into that lane's best-hit key (t bits, index) with an LDS
+    // 64-bit atomic min.  The minimum over the tests of one lane equals the
+    // sequential loop's result: a hit replaces the running best when
+    // (t, index) is lexicographically smaller, and with no best yet only a
+    // strictly smaller t qualifies (the initial key carries index 0).  With
+    // `any`, a lane whose key improved stops, as the loop stops at its first
+    // accepted hit (the caller then only needs that some hit was found).
+    if constexpr (CAT == C_TRI) {
+        // only where every lane traces the same scene (a batch's lanes may hold
+        // rays of different scenes, whose leaf arrays differ)
+        if (LEAF_BATCH && S.wave_one_scene && S.count[C_TRI] < (1 << LEAF_SLOT_BITS)) {
+            const uint64_t active = __ballot(1);
+            uint32_t n = 0;
+            for (uint32_t lm = leafmask; lm; lm &= lm - 1) n += ((uint32_t)(meta >> (8 * __builtin_ctz(lm))) & 0xFFu) >> 5;
+            const uint32_t lane = __lane_id();
+            const uint64_t lt = (1ull << lane) - 1ull;
+            uint32_t off = 0, N = 0;
+#pragma unroll
+            for (int b = 0; b < 6; b++) {
+                const uint64_t mb = __ballot((n >> b) & 1u);
+                off += (uint32_t)__popcll(mb & lt) << b;
+                N += (uint32_t)__popcll(mb) << b;
+            }
+            if (__ballot(n >= 2) != 0ull && N <= LEAF_LIST) {
+                for (uint32_t lm = leafmask, j = off; lm; lm &= lm - 1) {
+                    const uint32_t m = (uint32_t)(meta >> (8 * __builtin_ctz(lm))) & 0xFFu;
+                    for (uint32_t k = 0; k < (m >> 5); k++) st.list[j++] = (lane << LEAF_SLOT_BITS) | (prim_base + (m & 31u) + k);
+                }
+                const unsigned long long key0 =
+                    ((unsigned long long)__float_as_uint(T.ct) << 32) | (uint32_t)(T.ch ? T.ci : 0);
+                st.keys[lane] = key0;
+                if (COUNT) ts.prims += n;
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t A = (uint32_t)__popcll(active), q = (uint32_t)__popcll(active & lt);
+                for (uint32_t base = 0; base < N; base += A) {
+                    const uint32_t g = base + q;
+                    const uint32_t e = g < N ? st.list[g] : (lane << LEAF_SLOT_BITS);
+                    const int owner = (int)(e >> LEAF_SLOT_BITS);
+                    DRay ro;
+                    ro.o = mk3(__shfl(r.o.x, owner), __shfl(r.o.y, owner), __shfl(r.o.z, owner));
+                    ro.d = mk3(__shfl(r.d.x, owner), __shfl(r.d.y, owner), __shfl(r.d.z, owner));
+                    ro.tmin = __shfl(r.tmin, owner);
+                    ro.tmax = __shfl(r.tmax, owner);
+                    const v3 io = mk3(__shfl(inv.x, owner), __shfl(inv.y, owner), __shfl(inv.z, owner));
+                    if (g < N) {
+                        float t;
+                        int idx;
+                        if (leaf_dist<C_TRI>(S, e & ((1u << LEAF_SLOT_BITS) - 1u), ro, io, t, idx))
+                            atomicMin(&st.keys[owner], ((unsigned long long)__float_as_uint(t) << 32) | (uint32_t)idx);
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                const unsigned long long key = st.keys[lane];
+                if (key < key0) {
+                    T.ct = __uint_as_float((uint32_t)(key >> 32));
+                    T.ci = (int)(uint32_t)key;
+                    T.ch = true;
+                    if (any) return true;
+                }
+                leafmask = 0;
+            }
         }
     }
     while (leafmask) {

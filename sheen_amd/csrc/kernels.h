@@ -362,13 +362,17 @@ struct ManySrc {
 
 // Every kernel that traverses declares its LDS columns and its private
 // spill array, then builds the stack view over them.
-#define SHEEN_STACK(st)                                   \
-    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];     \
-    uint2 s_spill[SPILL_DEPTH];                           \
-    TStack st;                                            \
-    st.lds = s_stack + threadIdx.x;                       \
-    st.spill = s_spill;                                   \
-    st.spills = 0
+#define SHEEN_STACK(st)                                                   \
+    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];                     \
+    __shared__ uint32_t s_leaf_list[TRACE_BLOCK / 64][LEAF_LIST];          \
+    __shared__ unsigned long long s_leaf_keys[TRACE_BLOCK];               \
+    uint2 s_spill[SPILL_DEPTH];                                           \
+    TStack st;                                                            \
+    st.lds = s_stack + threadIdx.x;                                       \
+    st.spill = s_spill;                                                   \
+    st.spills = 0;                                                        \
+    st.list = s_leaf_list[threadIdx.x >> 6];                              \
+    st.keys = s_leaf_keys + (threadIdx.x & ~63u)
 
 // ---------------------------------------------------------------------------
 // Hit record of the wavefront buffers: {t bits, index, carried object id, kind}.

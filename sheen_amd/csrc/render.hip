@@ -436,6 +436,7 @@ static SheenContext* context_create(const SheenScene* in, int flags = 0) {
     for (int i = 0; i < d.num_lights; i++)
         if (in->lights[i].type == 3 && in->lights[i].beam > 0.0f) c->n_beam++;
     d.any_beams = c->n_beam > 0;
+    d.wave_one_scene = 1;  // a context's kernels trace only this scene; batch tables clear it
     d.any_strands = d.count[C_STRAND] > 0;
     c->full = d.count[C_STRAND] || d.count[C_HF] || d.count[C_SE] || d.count[C_LATHE] || d.count[C_PRISM] ||
               d.csg_root >= 0 || d.num_media > 0 || c->n_beam > 0;
@@ -1169,7 +1170,10 @@ SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* width
     b->base.push_back((uint32_t)slot0);
     b->pixels = px0;
     std::vector<DScene> table;
-    for (SheenContext* c : b->ctx) table.push_back(c->ds);
+    for (SheenContext* c : b->ctx) {
+        table.push_back(c->ds);
+        table.back().wave_one_scene = 0;  // a wave of the batch pipeline may hold rays of several scenes
+    }
     if (!alloc_dev((void**)&b->d_scenes, sizeof(DScene) * count) || !alloc_dev((void**)&b->d_frames, sizeof(FrameDesc) * count) ||
         !alloc_dev((void**)&b->d_base, sizeof(uint32_t) * (count + 1)) || !alloc_dev((void**)&b->frame, sizeof(float4) * px0) ||
         !alloc_dev((void**)&b->ctr, sizeof(Counters))) {
