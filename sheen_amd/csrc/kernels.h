@@ -322,6 +322,7 @@ struct OneSrc {
     SHD int find(uint32_t) const { return 0; }
     SHD int find_wave(uint32_t) const { return 0; }
     SHD const DScene& scene(int) const { return S; }
+    SHD const DScene& trace_scene(int) const { return S; }
     SHD const FrameDesc& frame(int) const { return F; }
     SHD uint32_t local(int, uint32_t i) const { return F.g0 + i; }
     SHD uint32_t pix_count() const { return F.n_slots / (uint32_t)F.ns; }
@@ -333,6 +334,7 @@ struct OneSrc {
 };
 struct ManySrc {
     const DScene* S;
+    const DScene* S1;  // the same descriptors with wave_one_scene set
     const FrameDesc* F;
     const uint32_t* base;  // [n + 1] first slot of each scene, then the total
     int n;
@@ -350,6 +352,12 @@ struct ManySrc {
     // every lane of the wave holds a slot of the same scene
     SHD int find_wave(uint32_t i) const { return __builtin_amdgcn_readfirstlane(find(i)); }
     SHD const DScene& scene(int k) const { return S[k]; }
+    // descriptor for the traversal: the one that lets the wave share its
+    // triangle tests when every active lane traces scene k (checked per call)
+    SHD const DScene& trace_scene(int k) const {
+        const int k0 = __builtin_amdgcn_readfirstlane(k);
+        return __ballot(k == k0) == __ballot(1) ? S1[k0] : S[k];
+    }
     SHD const FrameDesc& frame(int k) const { return F[k]; }
     SHD uint32_t local(int k, uint32_t i) const { return g0 + i - base[k]; }
     SHD uint32_t pix_count() const { return count; }
@@ -388,7 +396,7 @@ __global__ TRACE_BOUNDS void k_primary(SRC src, int4* hit_p, Counters* C) {
     const uint32_t n = src.n_slots();
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const int sid = src.find_wave(i);
-        const DScene& S = src.scene(sid);
+        const DScene& S = src.trace_scene(sid);
         const FrameDesc& F = src.frame(sid);
         int x, y, s;
         int4 out = make_int4(0, -1, 0, -1);
@@ -503,7 +511,7 @@ __global__ TRACE_BOUNDS void k_trace_sec(SRC src, const SecRay* q_in, int which,
     uint32_t nrefl = 0, nrefr = 0;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         SecRay e = q_in[j];
-        const DScene& S = src.scene(src.find(e.meta & 0x0FFFFFFFu));
+        const DScene& S = src.trace_scene(src.find(e.meta & 0x0FFFFFFFu));
         DRay r;
         r.o = mk3(e.ox, e.oy, e.oz);
         r.d = mk3(e.dx, e.dy, e.dz);
@@ -621,7 +629,7 @@ __global__ TRACE_BOUNDS void k_shadow(SRC src, EventBufs EB, Counters* C) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         ShadowReq q = EB.shadow_q[j];
         // request slot = event * L + light, event = sample slot * 3 + kind
-        const DScene& S = src.scene(src.find((q.slot & 0x7FFFFFFFu) / (uint32_t)EB.L / 3u));
+        const DScene& S = src.trace_scene(src.find((q.slot & 0x7FFFFFFFu) / (uint32_t)EB.L / 3u));
         float tr = shadow_eval<COUNT, FULL>(S, q, st, passes, ts);
         EB.trans[q.slot & 0x7FFFFFFFu] = tr;
     }

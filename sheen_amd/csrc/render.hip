@@ -1039,6 +1039,7 @@ struct SheenBatch {
     size_t pixels = 0;
     int L = 1;
     DScene* d_scenes = nullptr;
+    DScene* d_scenes1 = nullptr;  // copies with wave_one_scene set
     FrameDesc* d_frames = nullptr;
     uint32_t* d_base = nullptr;
     float4* frame = nullptr;
@@ -1058,7 +1059,7 @@ struct SheenBatch {
         for (SheenContext* c : ctx) delete c;
         wb.release();
         serial_wb.release();
-        void* ps[] = {d_scenes, d_frames, d_base, frame, ctr};
+        void* ps[] = {d_scenes, d_scenes1, d_frames, d_base, frame, ctr};
         for (void* p : ps) if (p) (void)hipFree(p);
         for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -1102,7 +1103,7 @@ static void batch_enqueue(SheenBatch* b) {
     SB.hit_p = w.hit_p; SB.samp = w.samp; SB.beam_add = nullptr; SB.media_p = nullptr; SB.media_r = nullptr; SB.n_beam = 0;
     for (const auto& ch : b->chunks) {
         ManySrc src;
-        src.S = b->d_scenes; src.F = b->d_frames; src.base = b->d_base; src.n = b->n;
+        src.S = b->d_scenes; src.S1 = b->d_scenes1; src.F = b->d_frames; src.base = b->d_base; src.n = b->n;
         src.g0 = ch.first; src.count = ch.second;
         const size_t n = ch.second;
         k_reset<<<1, 64, 0, s>>>(b->ctr);
@@ -1174,6 +1175,8 @@ SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* width
         table.push_back(c->ds);
         table.back().wave_one_scene = 0;  // a wave of the batch pipeline may hold rays of several scenes
     }
+    std::vector<DScene> table1(table);
+    for (DScene& d : table1) d.wave_one_scene = 1;  // for waves whose lanes all trace one scene
     if (!alloc_dev((void**)&b->d_scenes, sizeof(DScene) * count) || !alloc_dev((void**)&b->d_frames, sizeof(FrameDesc) * count) ||
         !alloc_dev((void**)&b->d_base, sizeof(uint32_t) * (count + 1)) || !alloc_dev((void**)&b->frame, sizeof(float4) * px0) ||
         !alloc_dev((void**)&b->ctr, sizeof(Counters))) {
@@ -1181,6 +1184,8 @@ SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* width
         return nullptr;
     }
     HIP_OR(hipMemcpy(b->d_scenes, table.data(), sizeof(DScene) * count, hipMemcpyHostToDevice), nullptr);
+    if (!alloc_dev((void**)&b->d_scenes1, sizeof(DScene) * count)) { sheen_set_error("batch allocation failed"); return nullptr; }
+    HIP_OR(hipMemcpy(b->d_scenes1, table1.data(), sizeof(DScene) * count, hipMemcpyHostToDevice), nullptr);
     HIP_OR(hipMemcpy(b->d_frames, b->frames.data(), sizeof(FrameDesc) * count, hipMemcpyHostToDevice), nullptr);
     HIP_OR(hipMemcpy(b->d_base, b->base.data(), sizeof(uint32_t) * (count + 1), hipMemcpyHostToDevice), nullptr);
     // 32M-slot chunks (within 16 GiB of work buffers) measured fastest on the 256-scene batch
