@@ -164,7 +164,7 @@ SHD bool leaf_test(const DScene& S, int cat, uint32_t slot, const DRay& r, v3 in
 // traversal is over (stack exhausted, or with `any` the first accepted hit).
 template <int CAT, bool COUNT>
 SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, TravState& T, TStack& st,
-                   TraceStats& ts) {
+                   TraceStats& ts, bool share = false) {
     const WNode* nodes;
     if constexpr (CAT == CAT_MESH) nodes = (cat == C_TRI) ? S.bvh[C_TRI] : S.bvh[C_SPHERE];
     else nodes = S.bvh[CAT];
@@ -251,9 +251,9 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
     // `any`, a lane whose key improved stops, as the loop stops at its first
     // accepted hit (the caller then only needs that some hit was found).
     if constexpr (CAT == C_TRI) {
-        // only where every lane traces the same scene (a batch's lanes may hold
+        // `share`: every lane traces the same scene (a batch's lanes may hold
         // rays of different scenes, whose leaf arrays differ)
-        if (LEAF_BATCH && S.wave_one_scene && S.count[C_TRI] < (1 << LEAF_SLOT_BITS)) {
+        if (LEAF_BATCH && share) {
             const uint64_t active = __ballot(1);
             uint32_t n = 0;
             for (uint32_t lm = leafmask; lm; lm &= lm - 1) n += ((uint32_t)(meta >> (8 * __builtin_ctz(lm))) & 0xFFu) >> 5;
@@ -367,7 +367,10 @@ SHD void traverse(const DScene& S, const DRay& r, v3 inv, float& cat_t, int& cat
                   TraceStats& ts) {
     TravState T;
     T.node = 0; T.sp = 0; T.ct = cat_t; T.ci = cat_idx; T.ch = cat_hit;
-    while (!trav_step<CAT, COUNT>(S, CAT, ANY, r, inv, T, st, ts)) {
+    // decided once per traversal: the scene descriptor may be a per-lane
+    // reference into global memory, read again on every node visit otherwise
+    const bool share = CAT == C_TRI && S.wave_one_scene && S.count[C_TRI] < (1 << LEAF_SLOT_BITS);
+    while (!trav_step<CAT, COUNT>(S, CAT, ANY, r, inv, T, st, ts, share)) {
     }
     cat_t = T.ct; cat_idx = T.ci; cat_hit = T.ch;
 }
