@@ -169,8 +169,18 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
     if constexpr (CAT == CAT_MESH) nodes = (cat == C_TRI) ? S.bvh[C_TRI] : S.bvh[C_SPHERE];
     else nodes = S.bvh[CAT];
     const uint4* np = reinterpret_cast<const uint4*>(nodes + T.node);
-    uint4 h0 = np[0], h1 = np[1];
-    uint4 q0 = np[2], q1 = np[3], q2 = np[4], q3 = np[5], q4 = np[6], q5 = np[7];
+    // Each axis's two plane rows in near/far order for this ray, chosen by the
+    // sign bit of inv in the load address (no per-node selects); the exact
+    // path below is symmetric in the two rows.
+    // The signs are packed into one word that is opaque to the compiler, so
+    // the six row offsets are derived per visit rather than hoisted out of
+    // the traversal loop and kept (spilled) across it.
+    uint32_t z = (__float_as_uint(inv.x) >> 31) | ((__float_as_uint(inv.y) >> 31) << 1) |
+                 ((__float_as_uint(inv.z) >> 31) << 2);
+    asm volatile("" : "+v"(z));
+    const uint32_t zx = z & 1u, zy = (z >> 1) & 1u, zz = z >> 2;
+    const uint4 h0 = np[0], h1 = np[1];
+    const uint4 nx = np[2 + zx], fx = np[3 - zx], ny = np[4 + zy], fy = np[5 - zy], nz = np[6 + zz], fz = np[7 - zz];
     if (COUNT) ts.nodes++;
     const float ox = __uint_as_float(h0.x), oy = __uint_as_float(h0.y), oz = __uint_as_float(h0.z);
     const float sx = __uint_as_float((h0.w & 0xFFu) << 23);
@@ -181,16 +191,25 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
     const float bound = T.ct + T.ct * CULL_MARGIN;
     uint32_t keys[8];
     uint32_t leafmask = 0;
-    auto g = [](uint4 row, int i) -> float {
-        const int w = i >> 1, sh = 16 * (i & 1);
-        const uint32_t d = (w == 0) ? row.x : (w == 1) ? row.y : (w == 2) ? row.z : row.w;
-        return (float)((d >> sh) & 0xFFFFu);
+    // child plane i of a row: the fp16 pattern of an integer 0..WBVH_QMAX
+    auto word = [](uint4 row, int i) -> uint32_t {
+        const int w = i >> 1;
+        return (w == 0) ? row.x : (w == 1) ? row.y : (w == 2) ? row.z : row.w;
     };
+    auto g = [&](uint4 row, int i) -> float {
+        const uint32_t d = word(row, i) >> (16 * (i & 1));
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)(d & 0xFFFFu));
+    };
+    // fmaf(plane, b, a) of an fp16 plane: one mixed-precision FMA
+    // (v_fma_mix_f32, the half taken straight from the word), product and sum
+    // fused and rounded once as fmaf of the exact float
+    auto gfma = [&](uint4 row, int i, float b, float a) -> float { return fmaf(g(row, i), b, a); };
     // Distance-space decode: a plane q of axis a lies at t = q*B_a + A_a with
     // B_a = s_a*inv_a (exact: s_a is a power of two) and A_a = (o_a - ray_a)*inv_a.
     // Rounding makes this differ from the exact-path value ((q*s+o) - ray)*inv
-    // by less than E_a = 2^-20 (|A_a| + 65535|B_a| + (|o_a| + 65535 s_a)|inv_a|)
-    // (the terms need about 7u|A| + 4u*65535|B| + u(|o|+65535s)|inv|, u = 2^-24),
+    // by less than E_a = 2^-20 (|A_a| + Q|B_a| + (|o_a| + Q s_a)|inv_a|), Q = 2048
+    // (the terms need about 7u|A| + 4uQ|B| + u(|o|+Qs)|inv|, u = 2^-24; the
+    // 2^-100 floor covers a result or operand flushed from the denormal range),
     // so near planes offset by -E_a and far planes by +E_a bound the exact-path
     // interval from outside and the test stays a superset.  The offset is per
     // axis: an axis the ray nearly parallels has huge |inv| and planes far off
@@ -199,24 +218,19 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
     // the whole wave.
     const float Ax = (ox - r.o.x) * inv.x, Ay = (oy - r.o.y) * inv.y, Az = (oz - r.o.z) * inv.z;
     const float Bx = sx * inv.x, By = sy * inv.y, Bz = sz * inv.z;
-    const float Ex = fmaf(fmaf(fmaf(65535.0f, sx, fabsf(ox)), fabsf(inv.x), fmaf(65535.0f, fabsf(Bx), fabsf(Ax))), 0x1p-20f, 0x1p-100f);
-    const float Ey = fmaf(fmaf(fmaf(65535.0f, sy, fabsf(oy)), fabsf(inv.y), fmaf(65535.0f, fabsf(By), fabsf(Ay))), 0x1p-20f, 0x1p-100f);
-    const float Ez = fmaf(fmaf(fmaf(65535.0f, sz, fabsf(oz)), fabsf(inv.z), fmaf(65535.0f, fabsf(Bz), fabsf(Az))), 0x1p-20f, 0x1p-100f);
+    const float Q = (float)WBVH_QMAX;
+    const float Ex = fmaf(fmaf(fmaf(Q, sx, fabsf(ox)), fabsf(inv.x), fmaf(Q, fabsf(Bx), fabsf(Ax))), 0x1p-20f, 0x1p-100f);
+    const float Ey = fmaf(fmaf(fmaf(Q, sy, fabsf(oy)), fabsf(inv.y), fmaf(Q, fabsf(By), fabsf(Ay))), 0x1p-20f, 0x1p-100f);
+    const float Ez = fmaf(fmaf(fmaf(Q, sz, fabsf(oz)), fabsf(inv.z), fmaf(Q, fabsf(Bz), fabsf(Az))), 0x1p-20f, 0x1p-100f);
     const float inf = __builtin_huge_valf();
     const bool finite = Ex < inf && Ey < inf && Ez < inf;  // false for NaN too
     if (BOX_FAST && __ballot(!finite) == 0ull) {
-        const bool px = inv.x >= 0.0f, py = inv.y >= 0.0f, pz = inv.z >= 0.0f;
-        const uint4 nx = px ? q0 : q1, fx = px ? q1 : q0;
-        const uint4 ny = py ? q2 : q3, fy = py ? q3 : q2;
-        const uint4 nz = pz ? q4 : q5, fz = pz ? q5 : q4;
         const float Anx = Ax - Ex, Afx = Ax + Ex, Any = Ay - Ey, Afy = Ay + Ey, Anz = Az - Ez, Afz = Az + Ez;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
-            const float tn = fmaxf(fmaxf(fmaxf(fmaf(g(nx, i), Bx, Anx), fmaf(g(ny, i), By, Any)), fmaf(g(nz, i), Bz, Anz)),
-                                   r.tmin);
-            const float tf = fminf(fminf(fminf(fmaf(g(fx, i), Bx, Afx), fmaf(g(fy, i), By, Afy)), fmaf(g(fz, i), Bz, Afz)),
-                                   bound);
+            const float tn = fmaxf(fmaxf(fmaxf(gfma(nx, i, Bx, Anx), gfma(ny, i, By, Any)), gfma(nz, i, Bz, Anz)), r.tmin);
+            const float tf = fminf(fminf(fminf(gfma(fx, i, Bx, Afx), gfma(fy, i, By, Afy)), gfma(fz, i, Bz, Afz)), bound);
             const bool hit = (m != WBVH_EMPTY) && (tn <= tf);
             const bool leaf = (m >> 5) != 0;
             keys[i] = (hit && !leaf) ? ((__float_as_uint(tn) & ~7u) | (m & 7u)) : 0xFFFFFFFFu;
@@ -226,9 +240,9 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
-            const float lx = fmaf(g(q0, i), sx, ox), hx = fmaf(g(q1, i), sx, ox);
-            const float ly = fmaf(g(q2, i), sy, oy), hy = fmaf(g(q3, i), sy, oy);
-            const float lz = fmaf(g(q4, i), sz, oz), hz = fmaf(g(q5, i), sz, oz);
+            const float lx = fmaf(g(nx, i), sx, ox), hx = fmaf(g(fx, i), sx, ox);
+            const float ly = fmaf(g(ny, i), sy, oy), hy = fmaf(g(fy, i), sy, oy);
+            const float lz = fmaf(g(nz, i), sz, oz), hz = fmaf(g(fz, i), sz, oz);
             const float ax = (lx - r.o.x) * inv.x, bx = (hx - r.o.x) * inv.x;
             const float ay = (ly - r.o.y) * inv.y, by = (hy - r.o.y) * inv.y;
             const float az = (lz - r.o.z) * inv.z, bz = (hz - r.o.z) * inv.z;

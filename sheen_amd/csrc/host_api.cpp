@@ -5,13 +5,9 @@
 #include <vector>
 
 #include "host_util.h"
+#include "wbvh_rules.h"
 
-static float decode(uint16_t q, uint8_t e, float o) {
-    uint32_t bits = (uint32_t)e << 23;
-    float s;
-    std::memcpy(&s, &bits, 4);
-    return std::fmaf((float)q, s, o);
-}
+static float decode(uint16_t q, uint8_t e, float o) { return std::fmaf(wb_q_value(q), wb_exp_scale(e), o); }
 
 // Builds category `cat` of a scene and checks the wide BVH: every primitive
 // is referenced exactly once and every decoded child box along its path

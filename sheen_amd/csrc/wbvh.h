@@ -13,9 +13,10 @@
 //   [24,32)  meta[8]: 0xFF empty slot; (cnt<<5)|off with cnt in 1..7 for a
 //            leaf of cnt primitives at prim_base+off; (0<<5)|k for an
 //            internal child at child_base+k
-//   [32,128) 16-bit quantised child boxes, SoA: lo_x[8] hi_x[8] lo_y[8]
-//            hi_y[8] lo_z[8] hi_z[8]
-// A child bound decodes as fmaf(float(q), scale, origin).  The builder checks
+//   [32,128) quantised child boxes, SoA: lo_x[8] hi_x[8] lo_y[8] hi_y[8]
+//            lo_z[8] hi_z[8]; each an integer q in 0..WBVH_QMAX held as the
+//            bit pattern of the fp16 number q (exact)
+// A child bound decodes as fmaf(q, scale, origin).  The builder checks
 // every decoded bound against the exact float bound (same fmaf on the host),
 // so a decoded child box always contains the boxes below it.
 #pragma once
@@ -24,6 +25,7 @@
 #define WBVH_WIDTH 8
 #define WBVH_EMPTY 0xFFu
 #define WBVH_MAX_LEAF 4
+#define WBVH_QMAX 2048  // largest child-plane integer (fp16-exact)
 
 struct __attribute__((aligned(128))) WNode {
     float ox, oy, oz;

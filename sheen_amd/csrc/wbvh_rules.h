@@ -1,7 +1,7 @@
 // wbvh_rules.h — the arithmetic of the wide-BVH build, shared by the host
 // builder (wbvh_build.cpp) and the device builder (wbvh_gpu.hip) so that both
 // make the same choices bit for bit: centroids, bin indices, SAH costs in
-// double, the area-greedy collapse and the 16-bit child-box quantisation.
+// double, the area-greedy collapse and the child-box quantisation.
 // Both sides compile with -ffp-contract=off, so every expression below rounds
 // the same way on the CPU and the GPU.
 #pragma once
@@ -99,12 +99,30 @@ WB_HD inline void wb_sah(const float* cmn, const float* cmx, BinF bins, CntF cnt
 WB_HD inline int wb_bin_of(float c, float lo, float k) { return wb_bin((c - lo) * k); }
 WB_HD inline float wb_bin_scale(float ext) { return (float)WBVH_BINS / ext; }
 
-// Exponent byte e (scale 2^(e-127)) such that extent / scale <= 65000:
-// 2^k is the power of two just above extent / 65000.
+// Child planes are integers 0..WBVH_QMAX stored as fp16 bit patterns (exact:
+// at most 11 significant bits), so the traversal feeds them to a
+// mixed-precision FMA without a conversion.
+WB_HD inline uint16_t wb_q_bits(uint32_t q) {
+    if (q == 0) return 0;
+    int e = 0;
+    while ((q >> (e + 1)) != 0) e++;  // floor(log2 q), q <= 2048
+    const uint32_t m = (e <= 10) ? (q << (10 - e)) : (q >> (e - 10));
+    return (uint16_t)(((uint32_t)(e + 15) << 10) | (m & 0x3FFu));
+}
+// Value of such a pattern (zero or a normal fp16).
+WB_HD inline float wb_q_value(uint16_t h) {
+    if (h == 0) return 0.0f;
+    const int e = (int)(h >> 10) - 15;
+    const uint32_t m = (h & 0x3FFu) | 0x400u;
+    return (e >= 10) ? (float)(m << (e - 10)) : (float)(m >> (10 - e));
+}
+
+// Exponent byte e (scale 2^(e-127)) such that extent / scale <= 2032:
+// 2^k is the power of two just above extent / 2032.
 WB_HD inline uint8_t wb_pick_exp(float ext) {
     if (!(ext > 0.0f)) return 1;
     if (!(ext <= 3.40282347e38f)) return 254;
-    const double d = (double)ext / 65000.0;  // normal: ext >= 2^-149
+    const double d = (double)ext / 2032.0;  // normal: ext >= 2^-149
     uint64_t bits;
     memcpy(&bits, &d, 8);
     const int k = (int)((bits >> 52) & 0x7FF) - 1022;  // d = f * 2^k, f in [0.5, 1)
@@ -122,8 +140,8 @@ WB_HD inline float wb_exp_scale(uint8_t e) {
 
 // Quantises nc child boxes into wn (origin, exponents, q).  A decoded bound
 // fmaf(q, scale, origin) always contains the float bound; an axis whose
-// children do not fit 16 bits at its exponent takes the next exponent.
-// Returns false when no exponent fits.
+// children do not fit the 0..WBVH_QMAX grid at its exponent takes the next
+// exponent.  Returns false when no exponent fits.
 WB_HD inline bool wb_quantize(WNode& wn, const WbBox* cb, int nc) {
     WbBox nb;
     wb_reset(nb);
@@ -139,20 +157,21 @@ WB_HD inline bool wb_quantize(WNode& wn, const WbBox* cb, int nc) {
             const float sc = wb_exp_scale(ex[a]);
             bool ok = true;
             for (int i = 0; i < WBVH_WIDTH; i++) {
-                if (i >= nc) { wn.q[2 * a][i] = 65535; wn.q[2 * a + 1][i] = 0; continue; }
+                if (i >= nc) { wn.q[2 * a][i] = wb_q_bits(WBVH_QMAX); wn.q[2 * a + 1][i] = 0; continue; }
                 double lo = floor(((double)cb[i].mn[a] - (double)org[a]) / (double)sc);
                 double hi = ceil(((double)cb[i].mx[a] - (double)org[a]) / (double)sc);
-                lo = fmin(fmax(lo, 0.0), 65536.0);  // also maps NaN to 0
-                hi = fmin(fmax(hi, 0.0), 65536.0);
+                lo = fmin(fmax(lo, 0.0), (double)(WBVH_QMAX + 1));  // also maps NaN to 0
+                hi = fmin(fmax(hi, 0.0), (double)(WBVH_QMAX + 1));
                 int64_t ql = (int64_t)lo, qh = (int64_t)hi;
                 while (ql > 0 && fmaf((float)ql, sc, org[a]) > cb[i].mn[a]) ql--;
-                while (qh < 65535 && fmaf((float)qh, sc, org[a]) < cb[i].mx[a]) qh++;
-                if (fmaf((float)ql, sc, org[a]) > cb[i].mn[a] || fmaf((float)qh, sc, org[a]) < cb[i].mx[a] || qh > 65535) {
+                while (qh < WBVH_QMAX && fmaf((float)qh, sc, org[a]) < cb[i].mx[a]) qh++;
+                if (ql > WBVH_QMAX || qh > WBVH_QMAX || fmaf((float)ql, sc, org[a]) > cb[i].mn[a] ||
+                    fmaf((float)qh, sc, org[a]) < cb[i].mx[a]) {
                     ok = false;
                     break;
                 }
-                wn.q[2 * a][i] = (uint16_t)ql;
-                wn.q[2 * a + 1][i] = (uint16_t)qh;
+                wn.q[2 * a][i] = wb_q_bits((uint32_t)ql);
+                wn.q[2 * a + 1][i] = wb_q_bits((uint32_t)qh);
             }
             if (ok) break;
             if (ex[a] >= 254) return false;

@@ -2,14 +2,17 @@
 the exact-path interval of every child (so the node test stays a superset
 of the reference slab test on primitive bounds).
 
-Float32 emulation: + - * are IEEE-rounded by numpy; fmaf(q, B, A) with a
-16-bit q is emulated as an exact float64 product plus a float64 sum rounded
-to float32 (double rounding can move a result by one ulp in rare halfway
-cases, far inside the 16x margin of the bound).
+Child planes are integers q in 0..2048 held as fp16 patterns; the
+traversal's mixed-precision FMA takes q exactly.  Float32 emulation: + - *
+are IEEE-rounded by numpy; fmaf(q, B, A) is emulated as an exact float64
+product plus a float64 sum rounded to float32 (double rounding can move a
+result by one ulp in rare halfway cases, far inside the 16x margin of the
+bound).
 """
 import numpy as np
 
 F = np.float32
+QMAX = 2048
 
 
 def fma32(a, b, c):
@@ -27,8 +30,8 @@ def exact_path(q_lo, q_hi, o, s, ro, inv):
 def distance_space(q_lo, q_hi, o, s, ro, inv):
     A = ((o - ro).astype(F) * inv).astype(F)
     B = (s * inv).astype(F)
-    e = fma32(fma32(F(65535.0) * np.ones_like(s), s, np.abs(o)), np.abs(inv),
-              fma32(F(65535.0) * np.ones_like(B), np.abs(B), np.abs(A)))
+    e = fma32(fma32(F(QMAX) * np.ones_like(s), s, np.abs(o)), np.abs(inv),
+              fma32(F(QMAX) * np.ones_like(B), np.abs(B), np.abs(A)))
     E = fma32(e, F(2.0 ** -20) * np.ones_like(e), F(2.0 ** -100) * np.ones_like(e))
     pos = inv >= 0
     qn = np.where(pos, q_lo, q_hi)
@@ -42,8 +45,8 @@ def _case(rng, n, scale):
     o = (rng.uniform(-1, 1, n) * scale).astype(F)
     exp = rng.integers(-30, 12, n)
     s = np.ldexp(np.ones(n), exp).astype(F)
-    q_lo = rng.integers(0, 65535, n).astype(F)
-    q_hi = np.minimum(q_lo + rng.integers(0, 65535, n), 65535).astype(F)
+    q_lo = rng.integers(0, QMAX + 1, n).astype(F)
+    q_hi = np.minimum(q_lo + rng.integers(0, QMAX + 1, n), QMAX).astype(F)
     d = rng.normal(size=n).astype(F)
     d = np.where(rng.random(n) < 0.1, (d * 1e-6).astype(F), d)
     inv = (F(1.0) / d).astype(F)
@@ -77,3 +80,10 @@ def test_margin_is_needed():
     B = (s * inv).astype(F)
     near = fma32(np.where(inv >= 0, q_lo, q_hi), B, A)
     assert np.any(near > en) and np.any(near < en)
+
+
+def test_plane_integers_are_exact_in_fp16():
+    """Every plane value 0..2048 survives the fp16 pattern the nodes store."""
+    q = np.arange(QMAX + 1)
+    h = q.astype(np.float16)
+    assert np.array_equal(h.astype(np.float64), q.astype(np.float64))
