@@ -219,7 +219,10 @@ SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* width
                                const int* aa_samples, int count);
 void sheen_batch_destroy(SheenBatch* batch);
 /* Launch shapes as sheen_context_set_param, plus "graph" (0 = launch the
- * kernels directly instead of replaying the captured graph). */
+ * kernels directly instead of replaying the captured graph) and
+ * "scene_table" (1..512, default 512: entries of the per-block LDS copy of
+ * the scenes' first slots that the kernels search to find a slot's scene;
+ * beyond that many scenes the search ends in global memory). */
 int sheen_batch_set_param(SheenBatch* batch, const char* name, int value);
 /* One frame of every scene.  stats: ray counts of the whole batch, frame_ms
  * (GPU time, hipEvents) and trace_ms (wall clock of the call). */

@@ -310,14 +310,23 @@ SHD void refract_from(const DScene& S, const DHit& gh, v3 d, bool entering, floa
 // ManySrc: a batch; the scenes' slot ranges are concatenated (each a
 // multiple of 64, so a wave's 64 slots belong to one scene), the chunk is
 // [g0, g0 + n) of that space, and a slot's scene is found by binary search
-// over the scenes' first slots.  Kernels use:
+// over the scenes' first slots: first over every step-th of them, copied
+// into LDS when the kernel starts (prepare), then over at most step of them
+// in global memory (step = 1 for batches of up to SCENE_LDS scenes).
+// Kernels use:
+//   prepare(lds)    first statement of every kernel (a block barrier for ManySrc);
 //   n_slots()       slots of the chunk;
 //   find(i)         scene of chunk slot i (i < n_slots());
 //   scene(k), frame(k), local(k, i)  the scene, its frame, i in its slot space;
 //   pix_count(), pix_first(p, i0)    k_accumulate's enumeration of pixels.
+#define SCENE_LDS 512
+#define SCENE_TABLE(src)                          \
+    __shared__ uint32_t s_scene_first[SCENE_LDS]; \
+    src.prepare(s_scene_first)
 struct OneSrc {
     DScene S;
     FrameDesc F;
+    SHD void prepare(uint32_t*) {}
     SHD uint32_t n_slots() const { return F.n_slots; }
     SHD int find(uint32_t) const { return 0; }
     SHD int find_wave(uint32_t) const { return 0; }
@@ -339,15 +348,30 @@ struct ManySrc {
     const uint32_t* base;  // [n + 1] first slot of each scene, then the total
     int n;
     uint32_t g0, count;    // the chunk
+    int cap;               // entries of the LDS table, 1..SCENE_LDS
+    const uint32_t* tab;   // LDS: base[j * step] for j < m
+    int step, m;
+    SHD void prepare(uint32_t* lds) {
+        step = (n + cap - 1) / cap;
+        m = (n + step - 1) / step;
+        for (int j = (int)threadIdx.x; j < m; j += (int)blockDim.x) lds[j] = base[j * step];
+        __syncthreads();
+        tab = lds;
+    }
     SHD uint32_t n_slots() const { return count; }
     SHD int find(uint32_t i) const {
         const uint32_t g = g0 + i;
-        int lo = 0, hi = n - 1;
+        int lo = 0, hi = m - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (base[mid] <= g) lo = mid; else hi = mid - 1;
+            if (tab[mid] <= g) lo = mid; else hi = mid - 1;
         }
-        return lo;
+        int a = lo * step, b = min(a + step - 1, n - 1);
+        while (a < b) {
+            const int mid = (a + b + 1) >> 1;
+            if (base[mid] <= g) a = mid; else b = mid - 1;
+        }
+        return a;
     }
     // every lane of the wave holds a slot of the same scene
     SHD int find_wave(uint32_t i) const { return __builtin_amdgcn_readfirstlane(find(i)); }
@@ -390,6 +414,7 @@ SHD int4 pack_hit(const QHit& q) {
 
 template <bool COUNT, bool FULL, class SRC>
 __global__ TRACE_BOUNDS void k_primary(SRC src, int4* hit_p, Counters* C) {
+    SCENE_TABLE(src);
     SHEEN_STACK(st);
     TraceStats ts{0, 0};
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -430,6 +455,7 @@ SHD QHit unpack_hit(int4 h) {
 
 template <bool FULL, class SRC>
 __global__ SHADE_BOUNDS void k_shade_primary(SRC src, SampleBufs SB, EventBufs EB, SecRay* sec_q, Counters* C) {
+    SCENE_TABLE(src);
     SHEEN_STACK(st);
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t passes = 0;
@@ -504,6 +530,7 @@ __global__ SHADE_BOUNDS void k_shade_primary(SRC src, SampleBufs SB, EventBufs E
 
 template <bool COUNT, bool FULL, class SRC>
 __global__ TRACE_BOUNDS void k_trace_sec(SRC src, const SecRay* q_in, int which, int4* hits, Counters* C) {
+    SCENE_TABLE(src);
     SHEEN_STACK(st);
     TraceStats ts{0, 0};
     const uint32_t n = C->sec_n[which];
@@ -533,6 +560,7 @@ __global__ TRACE_BOUNDS void k_trace_sec(SRC src, const SecRay* q_in, int which,
 template <bool FULL, class SRC>
 __global__ SHADE_BOUNDS void k_shade_sec(SRC src, SampleBufs SB, EventBufs EB, const SecRay* q_in, int which,
                                          const int4* hits, SecRay* q_out, int which_out, Counters* C) {
+    SCENE_TABLE(src);
     SHEEN_STACK(st);
     const uint32_t n = C->sec_n[which];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -621,6 +649,7 @@ __global__ SHADE_BOUNDS void k_shade_sec(SRC src, SampleBufs SB, EventBufs EB, c
 
 template <bool COUNT, bool FULL, class SRC>
 __global__ TRACE_BOUNDS void k_shadow(SRC src, EventBufs EB, Counters* C) {
+    SCENE_TABLE(src);
     SHEEN_STACK(st);
     TraceStats ts{0, 0};
     const uint32_t n = C->shadow_n;
@@ -644,6 +673,7 @@ __global__ TRACE_BOUNDS void k_shadow(SRC src, EventBufs EB, Counters* C) {
 // Per-pixel accumulation (reference render_kernel.cu:536-813 blend order).
 template <class SRC>
 __global__ __launch_bounds__(256) void k_accumulate(SRC src, SampleBufs SB, EventBufs EB, float4* frame) {
+    SCENE_TABLE(src);
     const uint32_t n_pix = src.pix_count();
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pix; p += stride) {

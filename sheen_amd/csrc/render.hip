@@ -1050,6 +1050,7 @@ struct SheenBatch {
     int primary_grid = 64, sec_grid = 4, shadow_grid = 128, shade_grid = 2, shade_sec_grid = 2;  // tuned on the 256-scene batch
     size_t chunk_slots = 0;
     bool use_graph = true, dirty = true;
+    int scene_table = SCENE_LDS;  // LDS entries of the slot -> scene search
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~SheenBatch() {
@@ -1104,6 +1105,7 @@ static void batch_enqueue(SheenBatch* b) {
     for (const auto& ch : b->chunks) {
         ManySrc src;
         src.S = b->d_scenes; src.S1 = b->d_scenes1; src.F = b->d_frames; src.base = b->d_base; src.n = b->n;
+        src.cap = b->scene_table; src.tab = nullptr; src.step = 1; src.m = 1;
         src.g0 = ch.first; src.count = ch.second;
         const size_t n = ch.second;
         k_reset<<<1, 64, 0, s>>>(b->ctr);
@@ -1208,7 +1210,10 @@ int sheen_batch_set_param(SheenBatch* b, const char* name, int value) {
     else if (n == "shade_grid") b->shade_grid = value;
     else if (n == "shade_sec_grid") b->shade_sec_grid = value;
     else if (n == "graph") b->use_graph = value != 0;
-    else if (n == "chunk") {
+    else if (n == "scene_table") {
+        if (value < 1 || value > SCENE_LDS) { sheen_set_error("scene_table must be 1..%d", SCENE_LDS); return -1; }
+        b->scene_table = value;
+    } else if (n == "chunk") {
         const size_t cap = value > 0 ? (size_t)value : std::min((size_t)32 << 20, ((size_t)16 << 30) / slot_bytes(b->L, 0, false, false));
         size_t unit = 64;
         for (const FrameDesc& F : b->frames) unit = std::max(unit, (size_t)64 * F.ns);

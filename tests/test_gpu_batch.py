@@ -60,6 +60,22 @@ def test_chunks_across_scene_boundaries(batch):
     b.set_param("chunk", 0)
 
 
+def test_scene_table_sizes(batch):
+    """The slot -> scene search gives the same images whether the scenes'
+    first slots all sit in the LDS table or only every step-th of them."""
+    b = batch["b"]
+    b.render()
+    ref = [b.image(k) for k in range(b.n)]
+    for cap in (1, 3, 7):  # steps 8, 3, 2 over the 8 scenes
+        b.set_param("scene_table", cap)
+        b.render()
+        for k in range(b.n):
+            assert np.array_equal(b.image(k), ref[k]), (cap, k)
+    b.set_param("scene_table", 512)
+    with pytest.raises(RuntimeError):
+        b.set_param("scene_table", 513)
+
+
 def test_batched_scene_matches_oracle(batch):
     from oracle import Oracle
     b = batch["b"]
