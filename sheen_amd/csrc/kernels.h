@@ -403,8 +403,8 @@ struct ManySrc {
     st.lds = s_stack + threadIdx.x;                                       \
     st.spill = s_spill;                                                   \
     st.spills = 0;                                                        \
-    st.list = s_leaf_list[threadIdx.x >> 6];                              \
-    st.keys = s_leaf_keys + (threadIdx.x & ~63u)
+    st.list = s_leaf_list[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]; \
+    st.keys = s_leaf_keys + __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u)
 
 // ---------------------------------------------------------------------------
 // Hit record of the wavefront buffers: {t bits, index, carried object id, kind}.
