@@ -162,9 +162,16 @@ SHD bool leaf_test(const DScene& S, int cat, uint32_t slot, const DRay& r, v3 in
 // leaf child hit, then the next node (nearest internal child, else the
 // nearest stacked node still within the bound).  Returns true when the
 // traversal is over (stack exhausted, or with `any` the first accepted hit).
-template <int CAT, bool COUNT>
+// A traversal either keeps the closest hit (NoLeafVisit) or hands every leaf
+// primitive it reaches to a visitor (LV::active), leaving the bound at the
+// ray's t_max so that every leaf whose box the segment meets is reached.
+struct NoLeafVisit {
+    static constexpr bool active = false;
+    SHD void operator()(uint32_t) {}
+};
+template <int CAT, bool COUNT, class LV = NoLeafVisit>
 SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, TravState& T, TStack& st,
-                   TraceStats& ts, bool share = false) {
+                   TraceStats& ts, bool share = false, LV* lv = nullptr) {
     const WNode* nodes;
     if constexpr (CAT == CAT_MESH) nodes = (cat == C_TRI) ? S.bvh[C_TRI] : S.bvh[C_SPHERE];
     else nodes = S.bvh[CAT];
@@ -318,6 +325,12 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
                 }
                 leafmask = 0;
             }
+        }
+    }
+    if constexpr (LV::active) {
+        for (; leafmask; leafmask &= leafmask - 1) {
+            const uint32_t m = (uint32_t)(meta >> (8 * __builtin_ctz(leafmask))) & 0xFFu;
+            for (uint32_t k = 0; k < (m >> 5); k++) (*lv)(prim_base + (m & 31u) + k);
         }
     }
     while (leafmask) {
@@ -790,10 +803,10 @@ SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
     }
 }
 
-// Optical depth of all strands along a shadow ray (reference bvh.h:928-960):
-// brute force over strands through the strand BVH's leaf order.
-SHD float strands_depth(const DScene& S, const DRay& r) {
-    if (S.count[C_STRAND] == 0) return 0.0f;
+// Optical depth of all strands along a shadow ray, every strand in index
+// order (the restatement of the reference's sum, bvh.h:928-960, that the
+// BVH-driven sum below reproduces).
+SHD float strands_depth_linear(const DScene& S, const DRay& r) {
     const v3 inv = mk3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
     float tau = 0.0f;
     for (int i = 0; i < S.count[C_STRAND]; i++) {
@@ -801,6 +814,79 @@ SHD float strands_depth(const DScene& S, const DRay& r) {
         if (!ref_slab(r, inv, bb, bb + 3)) continue;
         tau += strand_depth(r, S.strands[i], S.sweep_points, 6.0f);
         if (tau > 6.0f) return tau;
+    }
+    return tau;
+}
+
+// The strands' nonzero optical depths met along a ray, at most STRAND_HITS
+// of them, kept sorted by strand index (unrolled insertion, no indexed
+// registers); `over` when more were met.
+#define STRAND_HITS 4
+struct StrandTau {
+    static constexpr bool active = true;
+    const DScene* S;
+    const DRay* r;
+    v3 inv;
+    uint32_t n = 0;
+    bool over = false;
+    int id[STRAND_HITS];
+    float tau[STRAND_HITS];
+    SHD void operator()(uint32_t slot) {
+        const int i = (int)S->prim_ids[C_STRAND][slot];
+        const float* bb = S->bounds[C_STRAND] + 6 * (size_t)i;
+        if (!ref_slab(*r, inv, bb, bb + 3)) return;
+        const float d = strand_depth(*r, S->strands[i], S->sweep_points, 6.0f);
+        if (d == 0.0f) return;  // a zero term leaves every partial sum unchanged
+        if (n == STRAND_HITS) { over = true; return; }
+        int ci = i;
+        float cd = d;
+#pragma unroll
+        for (int k = 0; k < STRAND_HITS; k++) {
+            if ((uint32_t)k < n && id[k] > ci) {  // shift the larger entry up
+                const int ti = id[k];
+                const float td = tau[k];
+                id[k] = ci; tau[k] = cd;
+                ci = ti; cd = td;
+            }
+            if ((uint32_t)k == n) { id[k] = ci; tau[k] = cd; }
+        }
+        n++;
+    }
+};
+
+// Optical depth through the strands (reference strand_bvh_tau, bvh.h:928-960,
+// with sigma 6 and the stop past 6): the sum over the strands whose bounds
+// pass the slab test, in strand index order.  The strand BVH finds them; a
+// strand the ray passes outside adds exactly 0, so summing only the nonzero
+// terms in index order gives the index-order sum over all of them bit for
+// bit.  A ray that meets more than STRAND_HITS strands takes the linear sum.
+// The slab test (bvh.h:578-600) checks only the x/y interval against
+// [t_min, t_max] and the z interval against the x/y one, so it accepts boxes
+// that the ray's line meets outside [t_min, t_max] (and strand_depth then
+// counts from t = 0); the walk therefore tests the node boxes against the
+// whole line, which contains every box the slab test accepts.
+SHD float strands_depth(const DScene& S, const DRay& r, TStack& st) {
+    if (S.count[C_STRAND] == 0) return 0.0f;
+    StrandTau v;
+    v.S = &S;
+    v.r = &r;
+    v.inv = mk3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    DRay line = r;
+    line.tmin = -__builtin_huge_valf();
+    line.tmax = __builtin_huge_valf();
+    TravState T;
+    T.node = 0; T.sp = 0; T.ct = __builtin_huge_valf(); T.ci = 0; T.ch = false;
+    TraceStats ts{0, 0};
+    while (!trav_step<C_STRAND, false, StrandTau>(S, C_STRAND, false, line, v.inv, T, st, ts, false, &v)) {
+    }
+    if (v.over) return strands_depth_linear(S, r);
+    float tau = 0.0f;
+#pragma unroll
+    for (int k = 0; k < STRAND_HITS; k++) {
+        if ((uint32_t)k < v.n) {
+            tau += v.tau[k];
+            if (tau > 6.0f) return tau;
+        }
     }
     return tau;
 }

@@ -135,7 +135,7 @@ SHD float shadow_eval(const DScene& S, const ShadowReq& q, TStack& st, uint32_t&
         if (!through) { blocked = true; break; }
     }
     if (blocked) return 0.0f;
-    if (FULL && (q.slot & 0x80000000u)) return expf(-strands_depth(S, orig));
+    if (FULL && (q.slot & 0x80000000u)) return expf(-strands_depth(S, orig, st));
     return 1.0f;
 }
 
