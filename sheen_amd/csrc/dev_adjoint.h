@@ -207,13 +207,13 @@ __global__ SHADE_BOUNDS void k_adjoint(DScene S, FrameDesc F, const int4* hit_p,
             base = albedo_of(S, mat, in);
             dbase = mk3(mat.ambient, mat.ambient, mat.ambient);
         }
-        const uint32_t ev = i * 3u;
+        const uint32_t ev = ev_of(EB, i, 0);
         for (int l = 0; l < L; l++) {
             float dI = 0.0f;
             if (hit) {
                 const Light& light = S.lights[l];
                 const LightSample u = light_sample(S, light, 1.0f, seed ^ (uint32_t)(uint32_t(l) * 2654435761u), rec.point);
-                const float tr = u.skip ? 0.0f : EB.trans[(size_t)ev * EB.L + l];
+                const float tr = u.skip ? 0.0f : EB.trans[ev_light(EB, ev, l)];
                 if (tr != 0.0f) {
                     const float diff = fmaxf(dot(in.normal, u.Ld), 0.0f);
                     float sp = 0.0f;

@@ -306,16 +306,26 @@ struct ShadowReq {
     uint32_t slot;  // event light slot; bit 31: strand optical depth needed
 };
 
-// Per-event outputs of the wavefront.
+// Per-event outputs of the wavefront.  An event is one of a sample slot's
+// three shading points (kind 0 primary, 1 reflection, 2 refraction), stored
+// in kind-major planes of `cap` slots: event ev = kind * cap + slot, so the
+// lanes of a wave (consecutive slots) touch consecutive entries.  Per-light
+// entries are light-major planes of 3 * cap events: ev + l * 3 * cap; a
+// shadow request carries that index.
 struct EventBufs {
-    float4* base;     // [events]: unshadowed term (lit) or sky colour; w = event kind bits
-    float4* caustic;  // [events] or null
-    float4* contrib;  // [events * L]: shade_direct of each light
-    float* trans;     // [events * L]: transmittance, written by the shadow kernel
+    float4* base;     // [3 cap]: unshadowed term (lit) or sky colour; w = event kind bits
+    float4* caustic;  // [3 cap] or null
+    float4* contrib;  // [L][3 cap]: shade_direct of each light
+    float* trans;     // [L][3 cap]: transmittance, written by the shadow kernel
     ShadowReq* shadow_q;
     uint32_t* shadow_n;
-    int L;  // lights per event in contrib/trans (a batch: the most of any scene)
+    int L;         // lights per event in contrib/trans (a batch: the most of any scene)
+    uint32_t cap;  // sample slots of the work buffers (3 * cap * max(L, 1) < 2^31)
 };
+SHD uint32_t ev_of(const EventBufs& B, uint32_t slot, uint32_t kind) { return kind * B.cap + slot; }
+SHD uint32_t ev_light(const EventBufs& B, uint32_t ev, int l) { return ev + (uint32_t)l * 3u * B.cap; }
+// sample slot of a per-light entry (a shadow request's index)
+SHD uint32_t ev_slot(const EventBufs& B, uint32_t e) { return e % B.cap; }
 // event kind bits (stored in base.w as int bits)
 #define EV_NONE 0
 #define EV_SKY 1
@@ -428,15 +438,15 @@ SHD void shade_event(const DScene& S, bool active, const DHit& rec, v3 view_dir,
             }
         }
         if (has_light) {
-            B.contrib[(size_t)ev * L + i] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
-            B.trans[(size_t)ev * L + i] = 0.0f;
+            B.contrib[ev_light(B, ev, i)] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+            B.trans[ev_light(B, ev, i)] = 0.0f;
         }
         const uint32_t q = block_reserve(B.shadow_n, emit, false);
         if (emit) {
             ShadowReq r;
             r.ox = so.x; r.oy = so.y; r.oz = so.z; r.tmax = stmax;
             r.dx = sd.x; r.dy = sd.y; r.dz = sd.z;
-            r.slot = ((uint32_t)ev * (uint32_t)L + (uint32_t)i) | ((want_tau && !debug) ? 0x80000000u : 0u);
+            r.slot = ev_light(B, ev, i) | ((want_tau && !debug) ? 0x80000000u : 0u);
             B.shadow_q[q] = r;
         }
     }
@@ -468,15 +478,14 @@ SHD v3 event_value(const DScene& S, uint32_t ev, const EventBufs& B) {
     int kind = __float_as_int(b.w);
     v3 r = mk3(b.x, b.y, b.z);
     if (!(kind & EV_LIT)) return r;
-    const int L = B.L;
     if (kind & EV_DEBUG) {
         int li = kind >> 8;
-        return (B.trans[(size_t)ev * L + li] > 0.0f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
+        return (B.trans[ev_light(B, ev, li)] > 0.0f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
     }
     for (int i = 0; i < S.num_lights; i++) {
-        float tr = B.trans[(size_t)ev * L + i];
+        float tr = B.trans[ev_light(B, ev, i)];
         if (tr != 0.0f) {
-            float4 c = B.contrib[(size_t)ev * L + i];
+            float4 c = B.contrib[ev_light(B, ev, i)];
             r = r + mk3(c.x, c.y, c.z) * tr;
         }
     }

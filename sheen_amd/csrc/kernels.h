@@ -481,8 +481,8 @@ __global__ SHADE_BOUNDS void k_shade_primary(SRC src, SampleBufs SB, EventBufs E
         const bool hit = valid && q.kind >= 0;
         DHit rec;
         if (hit) rebuild_hit<FULL>(S, r, q, rec);
-        if (valid && !hit) sky_event(S, r.d, i * 3 + 0, EB);
-        shade_event(S, hit, rec, -r.d, hit ? rec.t * pa : 0.0f, seed, i * 3 + 0, EB);
+        if (valid && !hit) sky_event(S, r.d, ev_of(EB, i, 0), EB);
+        shade_event(S, hit, rec, -r.d, hit ? rec.t * pa : 0.0f, seed, ev_of(EB, i, 0), EB);
         float4 sp = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
         int flags = valid ? SF_VALID : 0;
         bool want_r = false, want_t = false;
@@ -608,12 +608,12 @@ __global__ SHADE_BOUNDS void k_shade_sec(SRC src, SampleBufs SB, EventBufs EB, c
         bool want_next = false;
         SecRay nx;
         if (valid && !refr) {  // reflection: render_kernel.cu:682-711
-            ev = i * 3 + 1;
+            ev = ev_of(EB, i, 1);
             if (hit) lit = true;
             else sky_event(S, r.d, ev, EB);
             if (FULL && SB.media_r) SB.media_r[i] = media_of(S, r, hit ? h.t : 1.0e9f, seed, st, passes);
         } else if (valid) {  // refraction face: render_kernel.cu:767-802
-            ev = i * 3 + 2;
+            ev = ev_of(EB, i, 2);
             const uint32_t face = e.meta >> 30;
             const bool entering = (e.meta >> 29) & 1u;
             float w = e.w;
@@ -657,8 +657,8 @@ __global__ TRACE_BOUNDS void k_shadow(SRC src, EventBufs EB, Counters* C) {
     uint32_t passes = 0;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         ShadowReq q = EB.shadow_q[j];
-        // request slot = event * L + light, event = sample slot * 3 + kind
-        const DScene& S = src.trace_scene(src.find((q.slot & 0x7FFFFFFFu) / (uint32_t)EB.L / 3u));
+        // the request's index of its (event, light) entry gives the sample slot
+        const DScene& S = src.trace_scene(src.find(ev_slot(EB, q.slot & 0x7FFFFFFFu)));
         float tr = shadow_eval<COUNT, FULL>(S, q, st, passes, ts);
         EB.trans[q.slot & 0x7FFFFFFFu] = tr;
     }
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(256) void k_accumulate(SRC src, SampleBufs SB, Even
             const uint32_t i = i0 + (uint32_t)s * 64u;
             const float4 sp = SB.samp[i];
             const int flags = __float_as_int(sp.w);
-            v3 c = event_value(S, i * 3 + 0, EB);
+            v3 c = event_value(S, ev_of(EB, i, 0), EB);
             for (int k = 0; k < SB.n_beam && SB.beam_add; k++) {
                 float4 b = SB.beam_add[(size_t)i * SB.n_beam + k];
                 c = c + mk3(b.x, b.y, b.z);
@@ -699,7 +699,7 @@ __global__ __launch_bounds__(256) void k_accumulate(SRC src, SampleBufs SB, Even
                 c = c * m.w + mk3(m.x, m.y, m.z);
             }
             if (flags & SF_REFL) {
-                v3 rc = event_value(S, i * 3 + 1, EB);
+                v3 rc = event_value(S, ev_of(EB, i, 1), EB);
                 if (SB.media_r) {
                     float4 m = SB.media_r[i];
                     rc = rc * m.w + mk3(m.x, m.y, m.z);
@@ -709,7 +709,7 @@ __global__ __launch_bounds__(256) void k_accumulate(SRC src, SampleBufs SB, Even
             }
             if (flags & SF_REFR) {
                 v3 through = mk3(0, 0, 0);
-                if (flags & SF_EV2) through = through + sp.z * event_value(S, i * 3 + 2, EB);
+                if (flags & SF_EV2) through = through + sp.z * event_value(S, ev_of(EB, i, 2), EB);
                 c = c * (1.0f - fminf(sp.y, 1.0f)) + through;
             }
             acc = acc + c;

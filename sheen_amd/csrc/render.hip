@@ -478,6 +478,10 @@ static bool ensure_bufs(WorkBufs& w, size_t slots, int L, int n_beam, bool caust
     w.release();
     const size_t ev = slots * 3;
     const size_t Lq = (size_t)std::max(L, 1);
+    if (ev * Lq >= ((size_t)1 << 31)) {  // per-light event entries are indexed by 31 bits
+        sheen_set_error("work buffers of %zu slots x %d lights exceed the event index range", slots, L);
+        return false;
+    }
     bool ok = alloc_dev((void**)&w.hit_p, slots * sizeof(int4)) && alloc_dev((void**)&w.samp, slots * sizeof(float4)) &&
               alloc_dev((void**)&w.ev_base, ev * sizeof(float4)) &&
               alloc_dev((void**)&w.contrib, ev * Lq * sizeof(float4)) &&
@@ -640,6 +644,7 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     EB.base = w.ev_base; EB.caustic = caustic ? w.ev_caustic : nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
     EB.shadow_q = w.shadow_q; EB.shadow_n = &c->ctr->shadow_n;
     EB.L = c->ds.num_lights;
+    EB.cap = (uint32_t)w.cap_slots;
     SampleBufs SB;
     SB.hit_p = w.hit_p; SB.samp = w.samp; SB.beam_add = c->n_beam ? w.beam_add : nullptr;
     SB.media_p = c->ds.num_media ? w.media_p : nullptr; SB.media_r = c->ds.num_media ? w.media_r : nullptr;
@@ -1099,7 +1104,7 @@ static void batch_enqueue(SheenBatch* b) {
     k_zero_counters<<<1, 64, 0, s>>>(b->ctr);
     EventBufs EB;
     EB.base = w.ev_base; EB.caustic = nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
-    EB.shadow_q = w.shadow_q; EB.shadow_n = &b->ctr->shadow_n; EB.L = b->L;
+    EB.shadow_q = w.shadow_q; EB.shadow_n = &b->ctr->shadow_n; EB.L = b->L; EB.cap = (uint32_t)w.cap_slots;
     SampleBufs SB;
     SB.hit_p = w.hit_p; SB.samp = w.samp; SB.beam_add = nullptr; SB.media_p = nullptr; SB.media_r = nullptr; SB.n_beam = 0;
     for (const auto& ch : b->chunks) {
