@@ -200,8 +200,10 @@ int sheen_context_set_chunk(SheenContext* ctx, int samples_per_chunk);
 /* Launch-shape parameters set by the autotuner: "primary_grid", "sec_grid",
  * "shadow_grid", "shade_grid", "shade_sec_grid" (blocks per CU of each
  * grid-stride kernel; 0 = one work item per thread) and "chunk" (samples per
- * chunk, 0 = automatic).  Returns 0, or -1 for an unknown name or a negative
- * value. */
+ * chunk, 0 = automatic); "overlap" (1: the primary shading's shadow rays are
+ * traced on a second stream while the secondary rays run; default 0: one
+ * shadow launch after them; the image is the same either way).  Returns 0, or -1 for an
+ * unknown name or a negative value. */
 int sheen_context_set_param(SheenContext* ctx, const char* name, int value);
 
 /* ---- Batches of scenes -------------------------------------------------------

@@ -26,10 +26,12 @@ SPACE = {
     "sec_grid": (4, 6, 8, 12, 16, 24),
     "shade_grid": (1, 2, 3, 4, 6, 8, 16),
     "shade_sec_grid": (1, 2, 4, 8, 16),
+    "overlap": (0, 1),
 }
 
 # the library's built-in launch shapes (tuned on the 1M-triangle 1080p frame)
-DEFAULTS = {"primary_grid": 64, "sec_grid": 6, "shadow_grid": 64, "shade_grid": 2, "shade_sec_grid": 2, "chunk": 0}
+DEFAULTS = {"primary_grid": 64, "sec_grid": 6, "shadow_grid": 64, "shade_grid": 2, "shade_sec_grid": 2, "chunk": 0,
+            "overlap": 0}
 
 
 def _score(name, st):
@@ -44,6 +46,7 @@ def _score(name, st):
         return st["shade_primary_ms"]
     if name == "shade_sec_grid":
         return st["shade_ms"] - st["shade_primary_ms"]
+    # "overlap" runs kernels concurrently: only the frame time tells
     return st["frame_ms"]
 
 

@@ -92,7 +92,8 @@ struct SecRay {
 struct Counters {
     uint32_t sec_n[4];       // secondary queues (ping-pong) sizes
     uint32_t shadow_n;
-    uint32_t pad[3];
+    uint32_t shadow_n1;      // shadow requests queued by the primary shading (k_snap_shadow)
+    uint32_t pad[2];
     unsigned long long shadow_passes;
     unsigned long long nodes, prims, snodes, sprims, pnodes, pprims;
     unsigned long long reflection, refraction, extra_shadow;
@@ -648,14 +649,16 @@ __global__ SHADE_BOUNDS void k_shade_sec(SRC src, SampleBufs SB, EventBufs EB, c
 }
 
 template <bool COUNT, bool FULL, class SRC>
-__global__ TRACE_BOUNDS void k_shadow(SRC src, EventBufs EB, Counters* C) {
+__global__ TRACE_BOUNDS void k_shadow(SRC src, EventBufs EB, Counters* C, int part) {
     SCENE_TABLE(src);
     SHEEN_STACK(st);
     TraceStats ts{0, 0};
-    const uint32_t n = C->shadow_n;
+    // part 0: every queued request; 1: those of the primary shading; 2: the rest
+    const uint32_t lo = (part == 2) ? C->shadow_n1 : 0u;
+    const uint32_t n = (part == 1) ? C->shadow_n1 : C->shadow_n;
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t passes = 0;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    for (uint32_t j = lo + blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         ShadowReq q = EB.shadow_q[j];
         // the request's index of its (event, light) entry gives the sample slot
         const DScene& S = src.trace_scene(src.find(ev_slot(EB, q.slot & 0x7FFFFFFFu)));
@@ -719,6 +722,10 @@ __global__ __launch_bounds__(256) void k_accumulate(SRC src, SampleBufs SB, Even
     }
 }
 
+// Marks the end of the primary shading's shadow requests in the queue.
+__global__ void k_snap_shadow(Counters* C) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) C->shadow_n1 = C->shadow_n;
+}
 __global__ void k_reset(Counters* C) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         for (int k = 0; k < 4; k++) C->sec_n[k] = 0;

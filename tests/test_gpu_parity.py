@@ -101,6 +101,28 @@ def test_launch_shape_does_not_change_the_image(cfg):
         assert np.array_equal(img, imgs[0])
 
 
+@pytest.mark.parametrize("cfg", [2, 101, 103, 104, 105])
+def test_side_stream_shadows_do_not_change_the_image(cfg):
+    """The primary shading's shadow requests traced on a second stream while
+    the secondary rays run give the same image and ray counts as one shadow
+    launch after them (the default), across chunk boundaries too."""
+    from sheen_amd import scenes, sheen
+    sc = scenes.generate(cfg, **({"detail": 40} if cfg == 2 else {}))
+    out = {}
+    with sheen.Context(sc.scene) as ctx:
+        for overlap in (0, 1):
+            ctx.set_param("overlap", overlap)
+            for chunk in (0, 64 * 4 * 5):
+                ctx.set_chunk(chunk)
+                img, st = ctx.render(160, 120, 4)
+                out[overlap, chunk] = (img, st)
+    ref, st0 = out[0, 0]
+    for key, (img, st) in out.items():
+        assert np.array_equal(img, ref), key
+        for k in ("shadow_rays", "reflection_rays", "refraction_rays", "total_rays"):
+            assert st[k] == st0[k], (key, k)
+
+
 def test_autotuner_picks_from_its_space_and_keeps_the_image():
     from sheen_amd import scenes, sheen, autotune
     sc = scenes.generate(2, detail=40)
