@@ -145,7 +145,13 @@ SHD bool hit_triangle(const DRay& r, const Triangle& tri, DHit& h) {
 }
 
 // Cone frustum segment (reference primitives/cone.h:11-73).
-SHD bool hit_cone(const DRay& r, v3 p0, float r0, v3 p1, float r1, DHit& h) {
+// `smax`: a caller that only takes hits at distances up to smax (inclusive)
+// passes it to skip a frustum whose axial coordinate tp = v + s*u, for s in
+// [t_min, smax], stays outside [0, L]: in float, s*u and then v + (s*u)
+// round monotonically, so tp over that range lies between its values at the
+// two ends, and no root there can pass the axial test below (a NaN end
+// never skips).  Hits the frustum could report beyond smax are not needed.
+SHD bool hit_cone(const DRay& r, v3 p0, float r0, v3 p1, float r1, DHit& h, float smax) {
     v3 axis = p1 - p0;
     float L2 = dot(axis, axis);
     if (L2 < 1e-12f) return false;
@@ -155,6 +161,10 @@ SHD bool hit_cone(const DRay& r, v3 p0, float r0, v3 p1, float r1, DHit& h) {
     v3 Q = r.o - p0;
     float u = dot(r.d, da);
     float v = dot(Q, da);
+    {
+        const float ta = v + r.tmin * u, tb = v + smax * u;
+        if ((ta < 0.0f && tb < 0.0f) || (ta > L && tb > L)) return false;
+    }
     v3 Dp = r.d - u * da;
     v3 Qp = Q - v * da;
     float A = dot(Dp, Dp) - k * k * u * u;
@@ -223,7 +233,7 @@ SHD bool hit_strand(const DRay& r, const SweepStrand& st, const SweepPoint* pts,
     }
     if (hit_cap(r, cv3(p[0].position), p[0].radius, normalize(cv3(p[0].position) - cv3(p[1].position)), tmp)) { SHEEN_TAKE() }
     for (int i = 0; i < count - 1; i++) {
-        if (hit_cone(r, cv3(p[i].position), p[i].radius, cv3(p[i + 1].position), p[i + 1].radius, tmp)) { SHEEN_TAKE() }
+        if (hit_cone(r, cv3(p[i].position), p[i].radius, cv3(p[i + 1].position), p[i + 1].radius, tmp, r.tmax)) { SHEEN_TAKE() }
         if (i < count - 2) {
             if (hit_cap(r, cv3(p[i + 1].position), p[i + 1].radius,
                         normalize(cv3(p[i + 2].position) - cv3(p[i + 1].position)), tmp)) { SHEEN_TAKE() }
@@ -470,7 +480,10 @@ SHD bool hit_hf(const DRay& r, const HeightField& hf, DHit& h) {
 }
 
 // Lathe: cone segments on the Y axis (reference primitives/lathe.h:14-55).
-SHD bool hit_lathe(const DRay& r, const Lathe& l, const LathePoint* pts, DHit& h) {
+// `smax`: only hits at distances up to smax are needed (a traversal's
+// running best; r.tmax for the full result); the closest hit at or below it
+// is the same as without the bound.
+SHD bool hit_lathe(const DRay& r, const Lathe& l, const LathePoint* pts, DHit& h, float smax) {
     if (l.point_count < 2) return false;
     bool any = false;
     float closest = r.tmax;
@@ -479,7 +492,8 @@ SHD bool hit_lathe(const DRay& r, const Lathe& l, const LathePoint* pts, DHit& h
         const LathePoint& p0 = pts[l.point_offset + i];
         const LathePoint& p1 = pts[l.point_offset + i + 1];
         if (p0.radius <= 0.0f && p1.radius <= 0.0f) continue;
-        if (hit_cone(r, mk3(0.0f, p0.y, 0.0f), p0.radius, mk3(0.0f, p1.y, 0.0f), p1.radius, tmp) && tmp.t < closest) {
+        if (hit_cone(r, mk3(0.0f, p0.y, 0.0f), p0.radius, mk3(0.0f, p1.y, 0.0f), p1.radius, tmp, fminf(smax, closest)) &&
+            tmp.t < closest) {
             tmp.point = r.o + tmp.t * r.d;
             h = tmp;
             h.material_id = l.material_id;

@@ -93,8 +93,9 @@ struct TraceStats {
 };
 
 // Leaf test of one primitive reference: reference gate, then the distance.
+// `smax`: the traversal's running best (hits beyond it cannot be taken).
 template <int CAT>
-SHD bool leaf_dist(const DScene& S, uint32_t slot, const DRay& r, v3 inv, float& t, int& idx) {
+SHD bool leaf_dist(const DScene& S, uint32_t slot, const DRay& r, v3 inv, float& t, int& idx, float smax) {
     if (CAT == C_TRI) {
         float4 a = S.tri_leaf[3 * slot], b = S.tri_leaf[3 * slot + 1], c = S.tri_leaf[3 * slot + 2];
         idx = __float_as_int(a.w);
@@ -122,7 +123,7 @@ SHD bool leaf_dist(const DScene& S, uint32_t slot, const DRay& r, v3 inv, float&
             if (!hit_se(r, S.ses[idx], h)) return false;
             break;
         case C_LATHE:
-            if (!hit_lathe(r, S.lathes[idx], S.lathe_points, h)) return false;
+            if (!hit_lathe(r, S.lathes[idx], S.lathe_points, h, smax)) return false;
             break;
         case C_PRISM:
             if (!hit_prism(r, S.prisms[idx], S.prism_points, h)) return false;
@@ -149,12 +150,12 @@ struct TravState {
 #define CAT_MESH (-1)
 
 template <int CAT>
-SHD bool leaf_test(const DScene& S, int cat, uint32_t slot, const DRay& r, v3 inv, float& t, int& idx) {
+SHD bool leaf_test(const DScene& S, int cat, uint32_t slot, const DRay& r, v3 inv, float& t, int& idx, float smax) {
     if constexpr (CAT == CAT_MESH) {
-        if (cat == C_TRI) return leaf_dist<C_TRI>(S, slot, r, inv, t, idx);
-        return leaf_dist<C_SPHERE>(S, slot, r, inv, t, idx);
+        if (cat == C_TRI) return leaf_dist<C_TRI>(S, slot, r, inv, t, idx, smax);
+        return leaf_dist<C_SPHERE>(S, slot, r, inv, t, idx, smax);
     } else {
-        return leaf_dist<CAT>(S, slot, r, inv, t, idx);
+        return leaf_dist<CAT>(S, slot, r, inv, t, idx, smax);
     }
 }
 
@@ -311,7 +312,7 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
                     if (g < N) {
                         float t;
                         int idx;
-                        if (leaf_dist<C_TRI>(S, e & ((1u << LEAF_SLOT_BITS) - 1u), ro, io, t, idx))
+                        if (leaf_dist<C_TRI>(S, e & ((1u << LEAF_SLOT_BITS) - 1u), ro, io, t, idx, ro.tmax))
                             atomicMin(&st.keys[owner], ((unsigned long long)__float_as_uint(t) << 32) | (uint32_t)idx);
                     }
                 }
@@ -342,7 +343,7 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
             float t;
             int idx;
             if (COUNT) ts.prims++;
-            if (leaf_test<CAT>(S, cat, prim_base + off + k, r, inv, t, idx)) {
+            if (leaf_test<CAT>(S, cat, prim_base + off + k, r, inv, t, idx, T.ct)) {
                 if (t < T.ct || (t == T.ct && T.ch && idx < T.ci)) {
                     T.ct = t;
                     T.ci = idx;
@@ -764,7 +765,7 @@ SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
             h.material_id = S.ses[q.idx].material_id; h.object_id = -(q.idx + 1);
             break;
         case C_LATHE:
-            if (FULL) hit_lathe(r, S.lathes[q.idx], S.lathe_points, h);
+            if (FULL) hit_lathe(r, S.lathes[q.idx], S.lathe_points, h, r.tmax);
             h.object_id = -(q.idx + 1);
             break;
         case C_PRISM:

@@ -1,0 +1,84 @@
+"""Lathes (reference primitives/lathe.h:14-95): surfaces of revolution about
+the y axis as chains of cone frusta, the centre ignored by the hit test and
+the bounds +-1e30 in x/z, so every ray whose y-slab passes tests every
+lathe.  The HIP traversal skips a frustum whose axial coordinate stays
+outside the segment over [t_min, running best]; primary hits (kind, index,
+distance) must still equal the oracle's bit for bit and radiance within
+1e-4, for lathes that overlap, widen and narrow, touch the axis, carry a
+negative-radius point, and sit at a centre off the origin, seen from above,
+from the side and at grazing angles."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(__file__))
+from util import PyScene, f3, material  # noqa: E402
+
+from sheen_amd import contract as K  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+CAMERAS = [  # position, forward, up
+    ((0.0, 1.5, 9.0), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0)),
+    ((0.0, 12.0, 0.5), (0.0, -0.99875, -0.0499376), (0.0, 0.0499376, -0.99875)),
+    ((7.0, 0.3, 7.0), (-0.7071068, 0.0, -0.7071068), (0.0, 1.0, 0.0)),
+]
+
+
+def _scene(cam):
+    rng = np.random.default_rng(5)
+    ps = PyScene()
+    ps.set("materials", K.Material, [material(color=(0.8, 0.7, 0.6), spec=0.0), material(color=(0.3, 0.6, 0.9))],
+           "num_materials")
+    pos, fwd, up = cam
+    c = ps.s.camera
+    right = np.cross(np.array(fwd), np.array(up))
+    c.position, c.forward, c.up, c.right = f3(*pos), f3(*fwd), f3(*up), f3(*right)
+    c.focal_length, c.sensor_width = 28.0, 36.0
+    L = K.Light()
+    L.type, L.position, L.color, L.intensity, L.radius, L.flags, L.texture_id = 1, f3(3, 9, 4), f3(1, 1, 1), 300.0, \
+        0.0, 1, -1
+    ps.set("lights", K.Light, [L], "num_lights")
+    p = K.Plane()
+    p.normal, p.d, p.material_id = f3(0, 1, 0), 2.0, 0
+    ps.set("planes", K.Plane, [p], "num_planes")
+    pts, lathes = [], []
+    for k in range(8):
+        la = K.Lathe()
+        la.center = f3(*(rng.uniform(-2, 2, 3) if k % 3 else (0.0, 0.0, 0.0)))
+        la.point_offset, la.material_id = len(pts), k % 2
+        n = int(rng.integers(6, 34))
+        y = np.sort(rng.uniform(-2.0, 4.0, n))
+        rad = np.abs(np.cumsum(rng.normal(0, 0.3, n))) + 0.2 * k
+        if k == 3:
+            rad[n // 2] = -0.4  # a negative profile radius
+        if k == 5:
+            rad[0] = 0.0  # closed at the bottom
+        for yy, rr in zip(y, rad):
+            lp = K.LathePoint()
+            lp.y, lp.radius = float(yy), float(rr)
+            pts.append(lp)
+        la.point_count = n
+        lathes.append(la)
+    ps.set("lathe_points", K.LathePoint, pts, "num_lathe_points")
+    ps.set("lathes", K.Lathe, lathes, "num_lathes")
+    return ps
+
+
+@pytest.mark.parametrize("cam", range(len(CAMERAS)))
+def test_lathe_hits_match_oracle(cam):
+    from oracle import Oracle
+    from sheen_amd import sheen
+    ps = _scene(CAMERAS[cam])
+    W, H, AA = 160, 120, 2
+    with sheen.Context(ps.ptr) as ctx:
+        img, _ = ctx.render(W, H, AA)
+        kind, idx, t = ctx.primary_hits(W, H, AA)
+    ref = Oracle(ps.ptr, threads=min(16, os.cpu_count() or 1)).render(W, H, AA, hits=True)
+    assert int(np.sum(ref["kind"] == 5)) > 2000  # lathe hits
+    assert np.array_equal(kind, ref["kind"]) and np.array_equal(idx, ref["index"])
+    assert np.array_equal(t.view(np.uint32), ref["t"].view(np.uint32))
+    rel = np.abs(img[..., :3] - ref["image"][..., :3]) / np.maximum(np.abs(ref["image"][..., :3]), 1e-3)
+    assert float(np.mean(rel.max(-1) > 1e-4)) <= 0.001, rel.max()
