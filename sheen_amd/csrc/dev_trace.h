@@ -114,7 +114,7 @@ SHD bool leaf_dist(const DScene& S, uint32_t slot, const DRay& r, v3 inv, float&
             return t_sphere(r, cv3(o.param1), o.param2, t);
         }
         case C_STRAND:
-            if (!hit_strand(r, S.strands[idx], S.sweep_points, h)) return false;
+            if (!hit_strand(r, S.strands[idx], S.sweep_points, h, smax)) return false;
             break;
         case C_HF:
             if (!hit_hf(r, S.hfs[idx], h)) return false;
@@ -749,7 +749,7 @@ SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
             break;
         }
         case C_STRAND:
-            if (FULL) hit_strand(r, S.strands[q.idx], S.sweep_points, h);
+            if (FULL) hit_strand(r, S.strands[q.idx], S.sweep_points, h, r.tmax);
             h.object_id = -(q.idx + 1);
             break;
         case C_TRI:
