@@ -357,14 +357,37 @@ SHD bool se_span(const DRay& r, const Superellipsoid& se, float t0, float t1, fl
     if (!fx) tx = t1;
     return te <= tx;
 }
-SHD bool hit_se(const DRay& r, const Superellipsoid& se, DHit& h) {
+// The entry distance se_span returns, found without the exit search: the
+// entry is the first inside sample (the entry bisection starts with lo = hi
+// and returns that sample), and te <= tx always holds (the exit lies at or
+// after the last sample before it), so se_span succeeds exactly when an entry
+// is found.  Samples are non-decreasing in distance, so once one lies beyond
+// `smax` (a traversal's running best, inclusive) no later entry can be taken.
+SHD bool se_entry(const DRay& r, const Superellipsoid& se, float t0, float t1, float smax, float& te) {
+    {
+        float tA, tB;
+        se_slab(r, se, tA, tB);
+        if (tB < t0 || tA > t1 || tB < tA) return false;
+        t0 = fmaxf(t0, tA);
+        t1 = fminf(t1, tB);
+    }
+    const float STEP = (t1 - t0) / 32.0f;
+    if (se_field(se, r.o + t0 * r.d) <= 1.0f) { te = t0; return true; }
+    for (int i = 1; i <= 32; i++) {
+        const float t = fminf(t0 + STEP * (float)i, t1);
+        if (t > smax) return false;
+        if (se_field(se, r.o + t * r.d) <= 1.0f) { te = t; return true; }
+    }
+    return false;
+}
+SHD bool hit_se(const DRay& r, const Superellipsoid& se, DHit& h, float smax) {
     float t0, t1;
     se_slab(r, se, t0, t1);
     if (t1 < r.tmin || t0 > r.tmax || t1 < t0) return false;
     t0 = fmaxf(t0, r.tmin);
     t1 = fminf(t1, r.tmax);
-    float te, tx;
-    if (!se_span(r, se, t0, t1, te, tx)) return false;
+    float te;
+    if (!se_entry(r, se, t0, t1, smax, te)) return false;
     v3 P = r.o + te * r.d;
     h.t = te;
     h.point = P;
