@@ -630,6 +630,16 @@ SHD void closest_cat(const DScene& S, const DRay& r, v3 inv, float init_t, QHit&
     }
 }
 
+// A CSG hit's record carries the leaf of the winning span (index in the low
+// 30 bits, leaf kind above: 0 sphere, 1 superellipsoid), which with the hit
+// distance fixes everything the shading needs: spans are copied whole, so a
+// span's material is always its leaf's (csg_eval).
+SHD int csg_hit_leaf(const CSpan& iv) { return (int)(((uint32_t)iv.kind << 30) | ((uint32_t)iv.leaf & 0x3FFFFFFFu)); }
+SHD int csg_leaf_material(const DScene& S, int packed) {
+    const int leaf = packed & 0x3FFFFFFF, kind = (int)((uint32_t)packed >> 30);
+    return kind == 0 ? S.objects[leaf].material_id : kind == 1 ? S.ses[leaf].material_id : 0;
+}
+
 // scene_intersect (reference integrator/raycast.h:14-70).  FULL = false
 // compiles only the sphere, triangle and plane categories (scenes without
 // other kinds use that instantiation).
@@ -658,7 +668,7 @@ SHD void query_closest(const DScene& S, const DRay& r, QHit& q, TStack& st, Trac
         CSpan iv;
         float t;
         if (csg_query(S, r, r.tmin, r.tmax, iv, t) && t < best) {
-            best = t; q.t = t; q.kind = C_CSG; q.idx = 0; q.carry = last_obj;
+            best = t; q.t = t; q.kind = C_CSG; q.idx = csg_hit_leaf(iv); q.carry = last_obj;
         }
     }
     for (int i = 0; i < S.num_planes; i++) {
@@ -698,7 +708,7 @@ SHD bool query_opaque(const DScene& S, const DRay& r, QHit& q, TStack& st, Trace
     if (FULL && S.csg_root >= 0) {
         CSpan iv;
         float t;
-        if (csg_query(S, r, r.tmin, r.tmax, iv, t)) { q.t = t; q.kind = C_CSG; q.idx = 0; q.carry = 0; return true; }
+        if (csg_query(S, r, r.tmin, r.tmax, iv, t)) { q.t = t; q.kind = C_CSG; q.idx = csg_hit_leaf(iv); q.carry = 0; return true; }
     }
     bool any = false;
     float closest = r.tmax;
@@ -721,13 +731,9 @@ SHD int hit_material(const DScene& S, const QHit& q, const DRay& r) {
         case C_SE: return S.ses[q.idx].material_id;
         case C_LATHE: return S.lathes[q.idx].material_id;
         case C_PRISM: return S.prisms[q.idx].material_id;
-        case C_CSG: {
+        case C_CSG:
             if (!FULL) return 0;
-            CSpan iv;
-            float t;
-            csg_query(S, r, r.tmin, r.tmax, iv, t);
-            return iv.mat;
-        }
+            return csg_leaf_material(S, q.idx);
         case C_PLANE: return S.planes[q.idx].material_id;
     }
     return 0;
@@ -774,17 +780,17 @@ SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
             break;
         case C_CSG: {
             if (!FULL) break;
-            CSpan iv;
-            float t;
-            csg_query(S, r, r.tmin, r.tmax, iv, t);
+            // the query's winning span, carried in the hit (csg_hit_leaf)
+            const float t = q.t;
+            const int leaf = q.idx & 0x3FFFFFFF, kind = (int)((uint32_t)q.idx >> 30);
             h.t = t;
-            h.material_id = iv.mat;
+            h.material_id = csg_leaf_material(S, q.idx);
             h.point = r.o + t * r.d;
-            if (iv.kind == 0) {
-                h.normal = normalize(h.point - cv3(S.objects[iv.leaf].param1));
+            if (kind == 0) {
+                h.normal = normalize(h.point - cv3(S.objects[leaf].param1));
                 h.tangent = mk3(0, 0, 0);
-            } else if (iv.kind == 1) {
-                h.normal = normalize(se_gradient(S.ses[iv.leaf], h.point));
+            } else if (kind == 1) {
+                h.normal = normalize(se_gradient(S.ses[leaf], h.point));
                 h.tangent = mk3(1, 0, 0);
             } else {
                 h.normal = mk3(0, 1, 0);

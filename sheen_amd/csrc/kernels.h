@@ -751,7 +751,7 @@ __global__ void k_export_hits(FrameDesc F, const int4* hit_p, int* kind, int* in
         const size_t o = ((size_t)y * F.width + x) * F.ns + s;
         const int4 h = hit_p[i];
         kind[o] = h.w;
-        index[o] = h.w < 0 ? -1 : h.y;
+        index[o] = h.w < 0 ? -1 : h.w == C_CSG ? 0 : h.y;  // a CSG hit reports index 0 (its record holds the leaf)
         t[o] = h.w < 0 ? 0.0f : __int_as_float(h.x);
     }
 }
