@@ -755,7 +755,7 @@ SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
             break;
         }
         case C_STRAND:
-            if (FULL) hit_strand(r, S.strands[q.idx], S.sweep_points, h, r.tmax);
+            if (FULL) hit_strand(r, S.strands[q.idx], S.sweep_points, h, q.t);
             h.object_id = -(q.idx + 1);
             break;
         case C_TRI:
@@ -771,7 +771,7 @@ SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
             h.material_id = S.ses[q.idx].material_id; h.object_id = -(q.idx + 1);
             break;
         case C_LATHE:
-            if (FULL) hit_lathe(r, S.lathes[q.idx], S.lathe_points, h, r.tmax);
+            if (FULL) hit_lathe(r, S.lathes[q.idx], S.lathe_points, h, q.t);  // frusta past the hit are not needed
             h.object_id = -(q.idx + 1);
             break;
         case C_PRISM:
