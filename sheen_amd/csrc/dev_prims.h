@@ -144,29 +144,9 @@ SHD bool hit_triangle(const DRay& r, const Triangle& tri, DHit& h) {
     return true;
 }
 
-// Cone frustum segment (reference primitives/cone.h:11-73).
-// `smax`: a caller that only takes hits at distances up to smax (inclusive)
-// passes it to skip a frustum whose axial coordinate tp = v + s*u, for s in
-// [t_min, smax], stays outside [0, L]: in float, s*u and then v + (s*u)
-// round monotonically, so tp over that range lies between its values at the
-// two ends, and no root there can pass the axial test below (a NaN end
-// never skips).  Hits the frustum could report beyond smax are not needed.
-SHD bool hit_cone(const DRay& r, v3 p0, float r0, v3 p1, float r1, DHit& h, float smax) {
-    v3 axis = p1 - p0;
-    float L2 = dot(axis, axis);
-    if (L2 < 1e-12f) return false;
-    float L = sqrtf(L2);
-    v3 da = axis / L;
-    float k = (r1 - r0) / L;
-    v3 Q = r.o - p0;
-    float u = dot(r.d, da);
-    float v = dot(Q, da);
-    {
-        const float ta = v + r.tmin * u, tb = v + smax * u;
-        if ((ta < 0.0f && tb < 0.0f) || (ta > L && tb > L)) return false;
-    }
-    v3 Dp = r.d - u * da;
-    v3 Qp = Q - v * da;
+// Cone frustum segment (reference primitives/cone.h:11-73): cone_solve holds
+// the roots of a frustum once its frame is set up (the rest of hit_cone).
+SHD bool cone_solve(const DRay& r, v3 Dp, v3 Qp, v3 da, float u, float v, float k, float r0, float L, DHit& h) {
     float A = dot(Dp, Dp) - k * k * u * u;
     float B = 2.0f * (dot(Dp, Qp) - k * r0 * u - k * k * v * u);
     float C = dot(Qp, Qp) - k * k * v * v - 2.0f * k * r0 * v - r0 * r0;
@@ -189,6 +169,52 @@ SHD bool hit_cone(const DRay& r, v3 p0, float r0, v3 p1, float r1, DHit& h, floa
         return true;
     }
     return false;
+}
+// `smax`: a caller that only takes hits at distances up to smax (inclusive)
+// passes it to skip a frustum whose axial coordinate tp = v + s*u, for s in
+// [t_min, smax], stays outside [0, L]: in float, s*u and then v + (s*u)
+// round monotonically, so tp over that range lies between its values at the
+// two ends, and no root there can pass the axial test below (a NaN end
+// never skips).  Hits the frustum could report beyond smax are not needed.
+SHD bool hit_cone(const DRay& r, v3 p0, float r0, v3 p1, float r1, DHit& h, float smax) {
+    v3 axis = p1 - p0;
+    float L2 = dot(axis, axis);
+    if (L2 < 1e-12f) return false;
+    float L = sqrtf(L2);
+    v3 da = axis / L;
+    float k = (r1 - r0) / L;
+    v3 Q = r.o - p0;
+    float u = dot(r.d, da);
+    float v = dot(Q, da);
+    {
+        const float ta = v + r.tmin * u, tb = v + smax * u;
+        if ((ta < 0.0f && tb < 0.0f) || (ta > L && tb > L)) return false;
+    }
+    v3 Dp = r.d - u * da;
+    v3 Qp = Q - v * da;
+    return cone_solve(r, Dp, Qp, da, u, v, k, r0, L, h);
+}
+// hit_cone for the frustum from (0, y0, 0) to (0, y1, 0) (lathe profiles).
+// The axis and its unit vector have exact zero x and z components (0 - 0,
+// then 0 / L), so the products with them are left out: every value below
+// equals hit_cone's up to the sign of a zero, which changes no comparison and
+// no nonzero result, so the roots and the accepted hit are the same.
+SHD bool hit_cone_y(const DRay& r, float y0, float r0, float y1, float r1, DHit& h, float smax) {
+    const float ay = y1 - y0;
+    const float L2 = ay * ay;
+    if (L2 < 1e-12f) return false;
+    const float L = sqrtf(L2);
+    const float day = ay / L;
+    const float k = (r1 - r0) / L;
+    const float Qy = r.o.y - y0;
+    const float u = r.d.y * day, v = Qy * day;
+    {
+        const float ta = v + r.tmin * u, tb = v + smax * u;
+        if ((ta < 0.0f && tb < 0.0f) || (ta > L && tb > L)) return false;
+    }
+    const v3 Dp = mk3(r.d.x, r.d.y - u * day, r.d.z);
+    const v3 Qp = mk3(r.o.x, Qy - v * day, r.o.z);
+    return cone_solve(r, Dp, Qp, mk3(0.0f, day, 0.0f), u, v, k, r0, L, h);
 }
 // Half sphere on the side of `axis` (reference primitives/cone.h:78-121);
 // leaves h.tangent untouched, as the reference does.
@@ -517,8 +543,7 @@ SHD bool hit_lathe(const DRay& r, const Lathe& l, const LathePoint* pts, DHit& h
         const LathePoint& p0 = pts[l.point_offset + i];
         const LathePoint& p1 = pts[l.point_offset + i + 1];
         if (p0.radius <= 0.0f && p1.radius <= 0.0f) continue;
-        if (hit_cone(r, mk3(0.0f, p0.y, 0.0f), p0.radius, mk3(0.0f, p1.y, 0.0f), p1.radius, tmp, fminf(smax, closest)) &&
-            tmp.t < closest) {
+        if (hit_cone_y(r, p0.y, p0.radius, p1.y, p1.radius, tmp, fminf(smax, closest)) && tmp.t < closest) {
             tmp.point = r.o + tmp.t * r.d;
             h = tmp;
             h.material_id = l.material_id;
