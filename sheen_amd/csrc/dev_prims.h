@@ -472,7 +472,26 @@ SHD float hf_height(const HeightField& hf, float x, float z) {
     t = fminf(t + detail - 0.03f, 1.0f);
     return t * hf.height;
 }
-SHD bool hit_hf(const DRay& r, const HeightField& hf, DHit& h) {
+// Attributes of a height-field hit at distance tt, point P = r.o + tt * r.d.
+SHD void hf_attrs(const DRay& r, const HeightField& hf, float tt, v3 P, DHit& h) {
+    h.t = tt;
+    h.point = P;
+    float e = fmaxf(hf.size_x, hf.size_z) / 256.0f;
+    float hxp = hf_height(hf, P.x + e, P.z);
+    float hxm = hf_height(hf, P.x - e, P.z);
+    float hzp = hf_height(hf, P.x, P.z + e);
+    float hzm = hf_height(hf, P.x, P.z - e);
+    h.normal = normalize(mk3(-(hxp - hxm) / (2.0f * e), 1.0f, -(hzp - hzm) / (2.0f * e)));
+    h.tangent = mk3(1, 0, 0);
+    h.uvx = P.x; h.uvy = P.z;
+    if (dot(h.normal, r.d) > 0.0f) h.normal = -h.normal;
+}
+// `smax`: only hits up to smax (inclusive) are needed: a crossing found at a
+// march step brackets the hit in [max(t - step, t0), t] (the bisection keeps
+// it there and the midpoint of values at or above a bound is at or above
+// it), and the steps only increase, so the march stops once that lower end
+// passes smax.  `attrs`: false when only the distance is needed (traversal).
+SHD bool hit_hf(const DRay& r, const HeightField& hf, DHit& h, float smax, bool attrs) {
     float bmnx = hf.center.x - 0.5f * hf.size_x, bmny = hf.center.y, bmnz = hf.center.z - 0.5f * hf.size_z;
     float bmxx = hf.center.x + 0.5f * hf.size_x, bmxy = hf.center.y + hf.height, bmxz = hf.center.z + 0.5f * hf.size_z;
     float t0, t1;
@@ -495,6 +514,11 @@ SHD bool hit_hf(const DRay& r, const HeightField& hf, DHit& h) {
     float t = t0;
     bool inside = false;
     for (int i = 0; i < 256; i++) {
+        {
+            float lo0 = t - step;
+            if (lo0 < t0) lo0 = t0;
+            if (lo0 > smax) break;
+        }
         v3 p = r.o + t * r.d;
         bool iy = (p.y <= hf_height(hf, p.x, p.z) + hf.center.y);
         if (iy != inside) {
@@ -508,17 +532,8 @@ SHD bool hit_hf(const DRay& r, const HeightField& hf, DHit& h) {
             float tt = 0.5f * (lo + hi);
             v3 P = r.o + tt * r.d;
             if (tt >= r.tmin && tt <= r.tmax) {
-                h.t = tt;
-                h.point = P;
-                float e = fmaxf(hf.size_x, hf.size_z) / 256.0f;
-                float hxp = hf_height(hf, P.x + e, P.z);
-                float hxm = hf_height(hf, P.x - e, P.z);
-                float hzp = hf_height(hf, P.x, P.z + e);
-                float hzm = hf_height(hf, P.x, P.z - e);
-                h.normal = normalize(mk3(-(hxp - hxm) / (2.0f * e), 1.0f, -(hzp - hzm) / (2.0f * e)));
-                h.tangent = mk3(1, 0, 0);
-                h.uvx = P.x; h.uvy = P.z;
-                if (dot(h.normal, r.d) > 0.0f) h.normal = -h.normal;
+                if (attrs) hf_attrs(r, hf, tt, P, h);
+                else h.t = tt;
                 return true;
             }
             break;

@@ -117,7 +117,7 @@ SHD bool leaf_dist(const DScene& S, uint32_t slot, const DRay& r, v3 inv, float&
             if (!hit_strand(r, S.strands[idx], S.sweep_points, h, smax)) return false;
             break;
         case C_HF:
-            if (!hit_hf(r, S.hfs[idx], h)) return false;
+            if (!hit_hf(r, S.hfs[idx], h, smax, false)) return false;
             break;
         case C_SE:
             if (!hit_se(r, S.ses[idx], h, smax)) return false;
@@ -763,7 +763,8 @@ SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
             h.material_id = S.triangles[q.idx].material_id; h.object_id = -(q.idx + 1);
             break;
         case C_HF:
-            if (FULL) hit_hf(r, S.hfs[q.idx], h);
+            // the traced distance; the march is not repeated
+            if (FULL) hf_attrs(r, S.hfs[q.idx], q.t, r.o + q.t * r.d, h);
             h.material_id = S.hfs[q.idx].material_id; h.object_id = -(q.idx + 1);
             break;
         case C_SE:
