@@ -406,14 +406,8 @@ SHD bool se_entry(const DRay& r, const Superellipsoid& se, float t0, float t1, f
     }
     return false;
 }
-SHD bool hit_se(const DRay& r, const Superellipsoid& se, DHit& h, float smax) {
-    float t0, t1;
-    se_slab(r, se, t0, t1);
-    if (t1 < r.tmin || t0 > r.tmax || t1 < t0) return false;
-    t0 = fmaxf(t0, r.tmin);
-    t1 = fminf(t1, r.tmax);
-    float te;
-    if (!se_entry(r, se, t0, t1, smax, te)) return false;
+// Attributes of a superellipsoid hit at distance te.
+SHD void se_attrs(const DRay& r, const Superellipsoid& se, float te, DHit& h) {
     v3 P = r.o + te * r.d;
     h.t = te;
     h.point = P;
@@ -421,6 +415,18 @@ SHD bool hit_se(const DRay& r, const Superellipsoid& se, DHit& h, float smax) {
     h.tangent = mk3(1, 0, 0);
     h.uvx = P.x; h.uvy = P.y;
     if (dot(h.normal, r.d) > 0.0f) h.normal = -h.normal;
+}
+// `attrs`: false when only the distance is needed (traversal).
+SHD bool hit_se(const DRay& r, const Superellipsoid& se, DHit& h, float smax, bool attrs) {
+    float t0, t1;
+    se_slab(r, se, t0, t1);
+    if (t1 < r.tmin || t0 > r.tmax || t1 < t0) return false;
+    t0 = fmaxf(t0, r.tmin);
+    t1 = fminf(t1, r.tmax);
+    float te;
+    if (!se_entry(r, se, t0, t1, smax, te)) return false;
+    if (attrs) se_attrs(r, se, te, h);
+    else h.t = te;
     return true;
 }
 

@@ -120,7 +120,7 @@ SHD bool leaf_dist(const DScene& S, uint32_t slot, const DRay& r, v3 inv, float&
             if (!hit_hf(r, S.hfs[idx], h, smax, false)) return false;
             break;
         case C_SE:
-            if (!hit_se(r, S.ses[idx], h, smax)) return false;
+            if (!hit_se(r, S.ses[idx], h, smax, false)) return false;
             break;
         case C_LATHE:
             if (!hit_lathe(r, S.lathes[idx], S.lathe_points, h, smax)) return false;
@@ -768,7 +768,7 @@ SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
             h.material_id = S.hfs[q.idx].material_id; h.object_id = -(q.idx + 1);
             break;
         case C_SE:
-            if (FULL) hit_se(r, S.ses[q.idx], h, r.tmax);
+            if (FULL) se_attrs(r, S.ses[q.idx], q.t, h);  // the traced entry; no new search
             h.material_id = S.ses[q.idx].material_id; h.object_id = -(q.idx + 1);
             break;
         case C_LATHE:
