@@ -110,7 +110,7 @@ def load(path):
 # frame time (the kernels are replayed from one captured graph, so per-kernel
 # timers do not apply).
 BATCH_SPACE = {
-    "chunk": (1 << 23, 1 << 24, 1 << 25),
+    "chunk": (1 << 24, 1 << 25, 1 << 26),
     "shadow_grid": (16, 32, 64, 128),
     "primary_grid": (8, 16, 64, 0),
     "sec_grid": (4, 8, 16, 32),

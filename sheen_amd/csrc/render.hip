@@ -1127,6 +1127,13 @@ static void batch_chunks(SheenBatch* b, size_t cap) {
 }
 
 // The batch frame's kernels on the batch stream (captured into the graph).
+// Default chunk of a batch: 64 M slots within 48 GiB of work buffers (an
+// MI355X holds 288 GB; on the 256-scene batch, 64 M-slot chunks measured 2 %
+// faster than 32 M-slot ones, fewer chunk tails).
+static size_t batch_default_chunk(int L) {
+    return std::min((size_t)64 << 20, ((size_t)48 << 30) / slot_bytes(L, 0, false, false));
+}
+
 static void batch_enqueue(SheenBatch* b) {
     hipStream_t s = b->stream;
     WorkBufs& w = b->wb;
@@ -1225,8 +1232,7 @@ SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* width
     HIP_OR(hipMemcpy(b->d_scenes1, table1.data(), sizeof(DScene) * count, hipMemcpyHostToDevice), nullptr);
     HIP_OR(hipMemcpy(b->d_frames, b->frames.data(), sizeof(FrameDesc) * count, hipMemcpyHostToDevice), nullptr);
     HIP_OR(hipMemcpy(b->d_base, b->base.data(), sizeof(uint32_t) * (count + 1), hipMemcpyHostToDevice), nullptr);
-    // 32M-slot chunks (within 16 GiB of work buffers) measured fastest on the 256-scene batch
-    b->chunk_slots = std::min((size_t)32 << 20, ((size_t)16 << 30) / slot_bytes(b->L, 0, false, false));
+    b->chunk_slots = batch_default_chunk(b->L);
     b->chunk_slots = std::min<size_t>(b->chunk_slots, ((size_t)slot0 + 63) / 64 * 64 + 64 * 64);
     batch_chunks(b.get(), b->chunk_slots);
     if (!ensure_bufs(b->wb, b->chunk_slots, b->L, 0, false, false)) return nullptr;
@@ -1249,7 +1255,8 @@ int sheen_batch_set_param(SheenBatch* b, const char* name, int value) {
         if (value < 1 || value > SCENE_LDS) { sheen_set_error("scene_table must be 1..%d", SCENE_LDS); return -1; }
         b->scene_table = value;
     } else if (n == "chunk") {
-        const size_t cap = value > 0 ? (size_t)value : std::min((size_t)32 << 20, ((size_t)16 << 30) / slot_bytes(b->L, 0, false, false));
+        size_t cap = value > 0 ? (size_t)value : batch_default_chunk(b->L);
+        if (value == 0) cap = std::min<size_t>(cap, ((size_t)b->base[b->n] + 63) / 64 * 64 + 64 * 64);  // no larger than the batch
         size_t unit = 64;
         for (const FrameDesc& F : b->frames) unit = std::max(unit, (size_t)64 * F.ns);
         if (cap < unit) { sheen_set_error("chunk smaller than a tile of the batch"); return -1; }
