@@ -397,6 +397,10 @@ static SheenContext* context_create(const SheenScene* in, int flags = 0) {
     d.prisms = c->upload(in->prisms, in->prisms ? (size_t)in->num_prisms : 0);
     d.prism_points = c->upload(in->prism_points, in->prism_points ? (size_t)in->num_prism_points : 0);
     d.csg = c->upload(in->csg_nodes, in->csg_nodes ? (size_t)in->num_csg_nodes : 0);
+    if (d.csg && (in->num_objects >= (1 << 30) || in->num_superellipsoids >= (1 << 30))) {
+        sheen_set_error("CSG leaves index at most 2^30 spheres or superellipsoids");  // hit records carry 30 bits
+        return nullptr;
+    }
     if (d.csg) {
         std::vector<float> cbox;
         host_csg_boxes(*in, cbox);
