@@ -30,7 +30,7 @@ SPACE = {
 }
 
 # the library's built-in launch shapes (tuned on the 1M-triangle 1080p frame)
-DEFAULTS = {"primary_grid": 64, "sec_grid": 6, "shadow_grid": 64, "shade_grid": 2, "shade_sec_grid": 2, "chunk": 0,
+DEFAULTS = {"primary_grid": 64, "sec_grid": 6, "shadow_grid": 64, "shade_grid": 1, "shade_sec_grid": 1, "chunk": 0,
             "overlap": 0}
 
 

@@ -171,7 +171,7 @@ SHD void wave_add(WaveTable& t, float* dst, bool active, uint32_t e, v3 val, boo
 // Gradient of sum_pixels <grad_img, pixel> for one chunk of primary samples.
 // Requires the chunk's primary hits (k_primary) and transmittances
 // (k_shade_primary + k_shadow) of the replayed frame.
-__global__ SHADE_BOUNDS void k_adjoint(DScene S, FrameDesc F, const int4* hit_p, EventBufs EB, const float* grad_img,
+__global__ ADJOINT_BOUNDS void k_adjoint(DScene S, FrameDesc F, const int4* hit_p, EventBufs EB, const float* grad_img,
                                        AdjGrads G) {
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t n_iter = (F.n_slots + stride - 1) / stride;  // whole waves reach every ballot

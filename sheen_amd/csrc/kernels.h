@@ -395,16 +395,21 @@ struct ManySrc {
 
 // Every kernel that traverses declares its LDS columns and its private
 // spill array, then builds the stack view over them.
-#define SHEEN_STACK(st)                                                   \
-    __shared__ uint2 s_stack[LDS_DEPTH * TRACE_BLOCK];                     \
-    __shared__ uint32_t s_leaf_list[TRACE_BLOCK / 64][LEAF_LIST];          \
-    __shared__ unsigned long long s_leaf_keys[TRACE_BLOCK];               \
-    uint2 s_spill[SPILL_DEPTH];                                           \
-    TStack st;                                                            \
-    st.lds = s_stack + threadIdx.x;                                       \
-    st.spill = s_spill;                                                   \
-    st.spills = 0;                                                        \
-    st.list = s_leaf_list[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]; \
+// SHEEN_STACK_N: the same for blocks of NT threads (a multiple of
+// TRACE_BLOCK), the LDS columns in groups of TRACE_BLOCK lanes so that every
+// lane's entries keep the TRACE_BLOCK stride the stack code assumes.
+#define SHEEN_STACK(st) SHEEN_STACK_N(st, TRACE_BLOCK)
+#define SHEEN_STACK_N(st, NT)                                                                    \
+    static_assert((NT) % TRACE_BLOCK == 0, "stack layout: blocks of TRACE_BLOCK lanes");        \
+    __shared__ uint2 s_stack[LDS_DEPTH * (NT)];                                                  \
+    __shared__ uint32_t s_leaf_list[(NT) / 64][LEAF_LIST];                                       \
+    __shared__ unsigned long long s_leaf_keys[NT];                                               \
+    uint2 s_spill[SPILL_DEPTH];                                                                  \
+    TStack st;                                                                                   \
+    st.lds = s_stack + (threadIdx.x / TRACE_BLOCK) * (LDS_DEPTH * TRACE_BLOCK) + threadIdx.x % TRACE_BLOCK; \
+    st.spill = s_spill;                                                                          \
+    st.spills = 0;                                                                               \
+    st.list = s_leaf_list[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];                     \
     st.keys = s_leaf_keys + __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u)
 
 // ---------------------------------------------------------------------------
@@ -457,7 +462,7 @@ SHD QHit unpack_hit(int4 h) {
 template <bool FULL, class SRC>
 __global__ SHADE_BOUNDS void k_shade_primary(SRC src, SampleBufs SB, EventBufs EB, SecRay* sec_q, Counters* C) {
     SCENE_TABLE(src);
-    SHEEN_STACK(st);
+    SHEEN_STACK_N(st, SHADE_BLOCK);
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t passes = 0;
     const uint32_t n = src.n_slots();
@@ -562,7 +567,7 @@ template <bool FULL, class SRC>
 __global__ SHADE_BOUNDS void k_shade_sec(SRC src, SampleBufs SB, EventBufs EB, const SecRay* q_in, int which,
                                          const int4* hits, SecRay* q_out, int which_out, Counters* C) {
     SCENE_TABLE(src);
-    SHEEN_STACK(st);
+    SHEEN_STACK_N(st, SHADE_BLOCK);
     const uint32_t n = C->sec_n[which];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t n_iter = (n + stride - 1) / stride;

@@ -88,7 +88,7 @@ struct SheenContext {
     long long chunk_override = 0;
     // launch shapes: blocks per CU of each grid-stride kernel (0 = one work
     // item per thread); queue-fed kernels are launched over the queue capacity
-    int primary_grid = 64, sec_grid = 6, shadow_grid = 64, shade_grid = 2, shade_sec_grid = 2, adjoint_grid = 8;
+    int primary_grid = 64, sec_grid = 6, shadow_grid = 64, shade_grid = 1, shade_sec_grid = 1, adjoint_grid = 8;
     // differentiable shading: why the scene is not differentiable ("" = it is),
     // float texel chains, their gradient, and the scene's own materials/lights
     std::string adj_reason;
@@ -162,15 +162,15 @@ struct SheenContext {
     }
 };
 
-static int grid_n(int num_cu, size_t n, int per_cu) {
-    size_t blocks = (n + TRACE_BLOCK - 1) / TRACE_BLOCK;
+static int grid_n(int num_cu, size_t n, int per_cu, int block = TRACE_BLOCK) {
+    size_t blocks = (n + block - 1) / block;
     size_t cap = per_cu > 0 ? (size_t)num_cu * per_cu : blocks;
     if (blocks > cap) blocks = cap;
     if (blocks > 0x7FFFFFFF) blocks = 0x7FFFFFFF;
     if (blocks < 1) blocks = 1;
     return (int)blocks;
 }
-static int grid_for(SheenContext* c, size_t n, int per_cu) { return grid_n(c->num_cu, n, per_cu); }
+static int grid_for(SheenContext* c, size_t n, int per_cu, int block = TRACE_BLOCK) { return grid_n(c->num_cu, n, per_cu, block); }
 
 static bool alloc_dev(void** p, size_t bytes) {
     *p = nullptr;
@@ -572,7 +572,7 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
         }
         {
             Timer t(c, 4);
-            k_shade_primary<FULL><<<grid_for(c, F.n_slots, c->shade_grid), TRACE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[0], c->ctr);
+            k_shade_primary<FULL><<<grid_for(c, F.n_slots, c->shade_grid, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[0], c->ctr);
         }
         if (adj) {
             // replay: shadow transmittances of the primary hits, then the adjoint
@@ -614,7 +614,7 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
             }
             {
                 Timer t(c, 3);
-                k_shade_sec<FULL><<<grid_for(c, cap, c->shade_sec_grid), TRACE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
+                k_shade_sec<FULL><<<grid_for(c, cap, c->shade_sec_grid, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
                                                                        w.sec_q[out_q], out_ctr, c->ctr);
             }
         }
@@ -1086,7 +1086,7 @@ struct SheenBatch {
     WorkBufs wb, serial_wb;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    int primary_grid = 64, sec_grid = 4, shadow_grid = 128, shade_grid = 2, shade_sec_grid = 2;  // tuned on the 256-scene batch
+    int primary_grid = 64, sec_grid = 4, shadow_grid = 128, shade_grid = 1, shade_sec_grid = 1;  // tuned on the 256-scene batch
     size_t chunk_slots = 0;
     bool use_graph = true, dirty = true;
     int scene_table = SCENE_LDS;  // LDS entries of the slot -> scene search
@@ -1156,12 +1156,12 @@ static void batch_enqueue(SheenBatch* b) {
         const size_t n = ch.second;
         k_reset<<<1, 64, 0, s>>>(b->ctr);
         k_primary<false, false><<<grid_n(cu, n, b->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, b->ctr);
-        k_shade_primary<false><<<grid_n(cu, n, b->shade_grid), TRACE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[0], b->ctr);
+        k_shade_primary<false><<<grid_n(cu, n, b->shade_grid, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[0], b->ctr);
         for (int round = 0; round < 3; round++) {
             const int in_q = round & 1, out_q = (round + 1) & 1;
             k_trace_sec<false, false><<<grid_n(cu, 2 * n, b->sec_grid), TRACE_BLOCK, 0, s>>>(src, w.sec_q[in_q], round,
                                                                                             w.sec_hits, b->ctr);
-            k_shade_sec<false><<<grid_n(cu, 2 * n, b->shade_sec_grid), TRACE_BLOCK, 0, s>>>(
+            k_shade_sec<false><<<grid_n(cu, 2 * n, b->shade_sec_grid, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(
                 src, SB, EB, w.sec_q[in_q], round, w.sec_hits, w.sec_q[out_q], round + 1, b->ctr);
         }
         k_shadow<false, false><<<grid_n(cu, 3 * n * (size_t)b->L, b->shadow_grid), TRACE_BLOCK, 0, s>>>(src, EB, b->ctr, 0);
