@@ -276,7 +276,7 @@ SHD uint32_t wave_reserve(uint32_t* counter, bool want) {
 // point (block-uniform control flow).  A thread's entries are base (if a) and
 // base + a (if b); returns 0xFFFFFFFF when it asks for none.
 SHD uint32_t block_reserve(uint32_t* counter, bool a, bool b) {
-    __shared__ uint32_t s_res[(SHADE_BLOCK > TRACE_BLOCK ? SHADE_BLOCK : TRACE_BLOCK) / 64 + 1];
+    __shared__ uint32_t s_res[1024 / 64 + 1];  // up to 1024-thread blocks
     const uint64_t ma = __ballot(a), mb = __ballot(b);
     const int lane = (int)__lane_id();
     const int wave = (int)(threadIdx.x >> 6);

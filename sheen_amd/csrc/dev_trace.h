@@ -34,21 +34,31 @@
 #else
 #define TRACE_BOUNDS __launch_bounds__(TRACE_BLOCK)
 #endif
-// The shading kernels run 512-thread blocks: their queue reservations take
-// one device atomic per block (block_reserve), and twice the threads per
-// atomic measured 0.4 ms faster on the 1M-triangle frame.  Their traversal
-// stack is laid out as halves of TRACE_BLOCK lanes (SHEEN_STACK_N), so the
-// stack code keeps its stride.
+// The shading kernels run 1024-thread blocks: their queue reservations take
+// one device atomic per block (block_reserve), and each light's shadow
+// requests of a block land in one contiguous run of the queue.  On the
+// 1M-triangle frame, against 256-thread blocks: shading 1.83 -> 1.37 ms and
+// the shadow kernel, tracing the longer coherent runs, 5.35 -> 4.95 ms (512:
+// 1.41 and 5.11 ms).  The instantiations for every primitive kind (FULL)
+// hold more registers and run 512-thread blocks (analytic 10k workload
+// 95.6 ms per frame against 98.8 ms with 1024).  Their traversal stack is laid
+// out in groups of TRACE_BLOCK lanes (SHEEN_STACK_N), so the stack code keeps
+// its stride.
 #ifndef SHADE_BLOCK
-#define SHADE_BLOCK 512
+#define SHADE_BLOCK 1024
 #endif
+#ifndef SHADE_BLOCK_FULL
+#define SHADE_BLOCK_FULL 512
+#endif
+#define SHADE_BLOCK_OF(FULL) ((FULL) ? SHADE_BLOCK_FULL : SHADE_BLOCK)
+static_assert(SHADE_BLOCK <= 1024 && SHADE_BLOCK_FULL <= 1024, "block_reserve holds up to 16 waves");
 #ifndef SHADE_MIN_WAVES
 #define SHADE_MIN_WAVES 0
 #endif
 #if SHADE_MIN_WAVES > 0
-#define SHADE_BOUNDS __launch_bounds__(SHADE_BLOCK, SHADE_MIN_WAVES)
+#define SHADE_BOUNDS(FULL) __launch_bounds__(SHADE_BLOCK_OF(FULL), SHADE_MIN_WAVES)
 #else
-#define SHADE_BOUNDS __launch_bounds__(SHADE_BLOCK)
+#define SHADE_BOUNDS(FULL) __launch_bounds__(SHADE_BLOCK_OF(FULL))
 #endif
 #define ADJOINT_BOUNDS __launch_bounds__(TRACE_BLOCK)
 #ifndef BOX_FAST

@@ -460,9 +460,9 @@ SHD QHit unpack_hit(int4 h) {
 }
 
 template <bool FULL, class SRC>
-__global__ SHADE_BOUNDS void k_shade_primary(SRC src, SampleBufs SB, EventBufs EB, SecRay* sec_q, Counters* C) {
+__global__ SHADE_BOUNDS(FULL) void k_shade_primary(SRC src, SampleBufs SB, EventBufs EB, SecRay* sec_q, Counters* C) {
     SCENE_TABLE(src);
-    SHEEN_STACK_N(st, SHADE_BLOCK);
+    SHEEN_STACK_N(st, SHADE_BLOCK_OF(FULL));
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t passes = 0;
     const uint32_t n = src.n_slots();
@@ -564,10 +564,10 @@ __global__ TRACE_BOUNDS void k_trace_sec(SRC src, const SecRay* q_in, int which,
 }
 
 template <bool FULL, class SRC>
-__global__ SHADE_BOUNDS void k_shade_sec(SRC src, SampleBufs SB, EventBufs EB, const SecRay* q_in, int which,
+__global__ SHADE_BOUNDS(FULL) void k_shade_sec(SRC src, SampleBufs SB, EventBufs EB, const SecRay* q_in, int which,
                                          const int4* hits, SecRay* q_out, int which_out, Counters* C) {
     SCENE_TABLE(src);
-    SHEEN_STACK_N(st, SHADE_BLOCK);
+    SHEEN_STACK_N(st, SHADE_BLOCK_OF(FULL));
     const uint32_t n = C->sec_n[which];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t n_iter = (n + stride - 1) / stride;

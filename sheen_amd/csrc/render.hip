@@ -572,7 +572,7 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
         }
         {
             Timer t(c, 4);
-            k_shade_primary<FULL><<<grid_for(c, F.n_slots, c->shade_grid, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[0], c->ctr);
+            k_shade_primary<FULL><<<grid_for(c, F.n_slots, c->shade_grid, SHADE_BLOCK_OF(FULL)), SHADE_BLOCK_OF(FULL), 0, s>>>(src, SB, EB, w.sec_q[0], c->ctr);
         }
         if (adj) {
             // replay: shadow transmittances of the primary hits, then the adjoint
@@ -614,7 +614,7 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
             }
             {
                 Timer t(c, 3);
-                k_shade_sec<FULL><<<grid_for(c, cap, c->shade_sec_grid, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
+                k_shade_sec<FULL><<<grid_for(c, cap, c->shade_sec_grid, SHADE_BLOCK_OF(FULL)), SHADE_BLOCK_OF(FULL), 0, s>>>(src, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
                                                                        w.sec_q[out_q], out_ctr, c->ctr);
             }
         }
