@@ -202,8 +202,12 @@ int sheen_context_set_chunk(SheenContext* ctx, int samples_per_chunk);
  * grid-stride kernel; 0 = one work item per thread) and "chunk" (samples per
  * chunk, 0 = automatic); "overlap" (1: the primary shading's shadow rays are
  * traced on a second stream while the secondary rays run; default 0: one
- * shadow launch after them; the image is the same either way).  Returns 0, or -1 for an
- * unknown name or a negative value. */
+ * shadow launch after them; the image is the same either way); "light_runs"
+ * (default 1: each light's shadow requests are one run of the queue, traced
+ * light after light; 0: one run shared by the lights, in shading order; a
+ * scene with primitives other than spheres, triangles and planes always uses
+ * one run; the image is the same either way).  Returns 0, or -1 for an unknown name or a
+ * negative value. */
 int sheen_context_set_param(SheenContext* ctx, const char* name, int value);
 
 /* ---- Batches of scenes -------------------------------------------------------

@@ -27,11 +27,12 @@ SPACE = {
     "shade_grid": (1, 2, 3, 4, 6, 8, 16),
     "shade_sec_grid": (1, 2, 4, 8, 16),
     "overlap": (0, 1),
+    "light_runs": (1, 0),
 }
 
 # the library's built-in launch shapes (tuned on the 1M-triangle 1080p frame)
 DEFAULTS = {"primary_grid": 64, "sec_grid": 6, "shadow_grid": 64, "shade_grid": 1, "shade_sec_grid": 1, "chunk": 0,
-            "overlap": 0}
+            "overlap": 0, "light_runs": 1}
 
 
 def _score(name, st):
@@ -46,7 +47,8 @@ def _score(name, st):
         return st["shade_primary_ms"]
     if name == "shade_sec_grid":
         return st["shade_ms"] - st["shade_primary_ms"]
-    # "overlap" runs kernels concurrently: only the frame time tells
+    # "overlap" runs kernels concurrently and "light_runs" reorders the shadow
+    # queue for the next shading too: only the frame time tells
     return st["frame_ms"]
 
 

@@ -318,7 +318,9 @@ struct EventBufs {
     float4* contrib;  // [L][3 cap]: shade_direct of each light
     float* trans;     // [L][3 cap]: transmittance, written by the shadow kernel
     ShadowReq* shadow_q;
-    uint32_t* shadow_n;
+    uint32_t* shadow_nl;  // [2 runs]: requests queued per run, then their count after the primary shading
+    uint32_t seg;         // queue entries per run (3 cap): run r's requests at r * seg
+    int runs;             // queue runs: L (light l's requests are run l) or 1 (one queue shared by the lights)
     int L;         // lights per event in contrib/trans (a batch: the most of any scene)
     uint32_t cap;  // sample slots of the work buffers (3 * cap * max(L, 1) < 2^31)
 };
@@ -441,7 +443,8 @@ SHD void shade_event(const DScene& S, bool active, const DHit& rec, v3 view_dir,
             B.contrib[ev_light(B, ev, i)] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
             B.trans[ev_light(B, ev, i)] = 0.0f;
         }
-        const uint32_t q = block_reserve(B.shadow_n, emit, false);
+        const int run = B.runs > 1 ? i : 0;
+        const uint32_t q = (uint32_t)run * B.seg + block_reserve(B.shadow_nl + run, emit, false);
         if (emit) {
             ShadowReq r;
             r.ox = so.x; r.oy = so.y; r.oz = so.z; r.tmax = stmax;

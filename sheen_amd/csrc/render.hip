@@ -56,13 +56,14 @@ struct WorkBufs {
     float4* contrib = nullptr;
     float* trans = nullptr;
     ShadowReq* shadow_q = nullptr;
+    uint32_t* shadow_nl = nullptr;  // per-light request counts and their snapshot
     SecRay* sec_q[2] = {nullptr, nullptr};
     int4* sec_hits = nullptr;
     float4* beam_add = nullptr;
     float4* media_p = nullptr;
     float4* media_r = nullptr;
     void release() {
-        void* ps[] = {hit_p, samp, ev_base, ev_caustic, contrib, trans, shadow_q, sec_q[0], sec_q[1], sec_hits,
+        void* ps[] = {hit_p, samp, ev_base, ev_caustic, contrib, trans, shadow_q, shadow_nl, sec_q[0], sec_q[1], sec_hits,
                       beam_add, media_p, media_r};
         for (void* p : ps) if (p) (void)hipFree(p);
         *this = WorkBufs();
@@ -107,6 +108,10 @@ struct SheenContext {
     hipStream_t side = nullptr;
     hipEvent_t ev_side[2] = {nullptr, nullptr};
     bool overlap = false;  // measured faster on the analytic 10k workload only
+    // shadow queue in one run per light (faster on the 1M-triangle frame) or
+    // one run shared by the lights; scenes shaded by the FULL kernels always
+    // use one run
+    bool light_runs = true;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     struct Mark { int slot; hipEvent_t a, b; };
@@ -502,6 +507,7 @@ static bool ensure_bufs(WorkBufs& w, size_t slots, int L, int n_beam, bool caust
               alloc_dev((void**)&w.contrib, ev * Lq * sizeof(float4)) &&
               alloc_dev((void**)&w.trans, ev * Lq * sizeof(float)) &&
               alloc_dev((void**)&w.shadow_q, ev * Lq * sizeof(ShadowReq)) &&
+              alloc_dev((void**)&w.shadow_nl, 2 * Lq * sizeof(uint32_t)) &&
               alloc_dev((void**)&w.sec_q[0], 2 * slots * sizeof(SecRay)) &&
               alloc_dev((void**)&w.sec_q[1], 2 * slots * sizeof(SecRay)) &&
               alloc_dev((void**)&w.sec_hits, 2 * slots * sizeof(int4));
@@ -559,7 +565,7 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
     OneSrc src;
     src.S = c->ds;
     src.F = F;
-        k_reset<<<1, 64, 0, s>>>(c->ctr);
+        k_reset<<<1, 64, 0, s>>>(c->ctr, EB.shadow_nl, EB.runs);
         {
             Timer t(c, 0);
             if (c->counting) k_primary<true, FULL><<<grid_for(c, F.n_slots, c->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, c->ctr);
@@ -585,10 +591,10 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
         }
         // The primary shading's shadow requests are complete here: trace them
         // on the side stream while the secondary rounds run on this one (their
-        // requests queue after shadow_n1; the two write disjoint entries).
+        // requests queue after each light's mark; the two write disjoint entries).
         const bool overlap = c->overlap;
         if (overlap) {
-            k_snap_shadow<<<1, 64, 0, s>>>(c->ctr);
+            k_snap_shadow<<<1, 64, 0, s>>>(EB.shadow_nl, EB.runs);
             (void)hipEventRecord(c->ev_side[0], s);
             (void)hipStreamWaitEvent(c->side, c->ev_side[0], 0);
             Timer t(c, 2, c->side);
@@ -675,8 +681,9 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     WorkBufs& w = *c->wbp;
     EventBufs EB;
     EB.base = w.ev_base; EB.caustic = caustic ? w.ev_caustic : nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
-    EB.shadow_q = w.shadow_q; EB.shadow_n = &c->ctr->shadow_n;
+    EB.shadow_q = w.shadow_q; EB.shadow_nl = w.shadow_nl; EB.seg = 3u * (uint32_t)w.cap_slots;
     EB.L = c->ds.num_lights;
+    EB.runs = (c->light_runs && !c->full) ? std::max(EB.L, 1) : 1;  // k_shadow<*, true> reads one run
     EB.cap = (uint32_t)w.cap_slots;
     SampleBufs SB;
     SB.hit_p = w.hit_p; SB.samp = w.samp; SB.beam_add = c->n_beam ? w.beam_add : nullptr;
@@ -854,6 +861,7 @@ int sheen_context_set_param(SheenContext* ctx, const char* name, int value) {
     const std::string n(name);
     if (value < 0) { sheen_set_error("negative parameter value"); return -1; }
     if (n == "overlap") ctx->overlap = value != 0;
+    else if (n == "light_runs") ctx->light_runs = value != 0;
     else if (n == "primary_grid") ctx->primary_grid = value;
     else if (n == "sec_grid") ctx->sec_grid = value;
     else if (n == "shadow_grid") ctx->shadow_grid = value;
@@ -1145,7 +1153,9 @@ static void batch_enqueue(SheenBatch* b) {
     k_zero_counters<<<1, 64, 0, s>>>(b->ctr);
     EventBufs EB;
     EB.base = w.ev_base; EB.caustic = nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
-    EB.shadow_q = w.shadow_q; EB.shadow_n = &b->ctr->shadow_n; EB.L = b->L; EB.cap = (uint32_t)w.cap_slots;
+    EB.shadow_q = w.shadow_q; EB.shadow_nl = w.shadow_nl; EB.seg = 3u * (uint32_t)w.cap_slots; EB.L = b->L;
+    EB.runs = std::max(b->L, 1);
+    EB.cap = (uint32_t)w.cap_slots;
     SampleBufs SB;
     SB.hit_p = w.hit_p; SB.samp = w.samp; SB.beam_add = nullptr; SB.media_p = nullptr; SB.media_r = nullptr; SB.n_beam = 0;
     for (const auto& ch : b->chunks) {
@@ -1154,7 +1164,7 @@ static void batch_enqueue(SheenBatch* b) {
         src.cap = b->scene_table; src.tab = nullptr; src.step = 1; src.m = 1;
         src.g0 = ch.first; src.count = ch.second;
         const size_t n = ch.second;
-        k_reset<<<1, 64, 0, s>>>(b->ctr);
+        k_reset<<<1, 64, 0, s>>>(b->ctr, EB.shadow_nl, EB.runs);
         k_primary<false, false><<<grid_n(cu, n, b->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, b->ctr);
         k_shade_primary<false><<<grid_n(cu, n, b->shade_grid, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[0], b->ctr);
         for (int round = 0; round < 3; round++) {

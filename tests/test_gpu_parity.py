@@ -105,18 +105,22 @@ def test_launch_shape_does_not_change_the_image(cfg):
 def test_side_stream_shadows_do_not_change_the_image(cfg):
     """The primary shading's shadow requests traced on a second stream while
     the secondary rays run give the same image and ray counts as one shadow
-    launch after them (the default), across chunk boundaries too."""
+    launch after them (the default), across chunk boundaries too, and so does
+    the shadow queue kept in one run per light (the default) or one run shared
+    by the lights."""
     from sheen_amd import scenes, sheen
     sc = scenes.generate(cfg, **({"detail": 40} if cfg == 2 else {}))
     out = {}
     with sheen.Context(sc.scene) as ctx:
-        for overlap in (0, 1):
-            ctx.set_param("overlap", overlap)
-            for chunk in (0, 64 * 4 * 5):
-                ctx.set_chunk(chunk)
-                img, st = ctx.render(160, 120, 4)
-                out[overlap, chunk] = (img, st)
-    ref, st0 = out[0, 0]
+        for runs in (1, 0):
+            ctx.set_param("light_runs", runs)
+            for overlap in (0, 1):
+                ctx.set_param("overlap", overlap)
+                for chunk in (0, 64 * 4 * 5):
+                    ctx.set_chunk(chunk)
+                    img, st = ctx.render(160, 120, 4)
+                    out[runs, overlap, chunk] = (img, st)
+    ref, st0 = out[1, 0, 0]
     for key, (img, st) in out.items():
         assert np.array_equal(img, ref), key
         for k in ("shadow_rays", "reflection_rays", "refraction_rays", "total_rays"):
