@@ -68,3 +68,26 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def _main(argv=None):
+    """`python -m sheen_amd.batch [batch|serial] [reps] [scenes]`: renders the
+    generated uneven batch (256 scenes by default) `reps` times, as one
+    pipeline or one render per scene, printing the GPU milliseconds of each
+    (a workload to run under a profiler)."""
+    import argparse
+    from . import scenes as gen
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("mode", nargs="?", default="batch", choices=("batch", "serial"))
+    ap.add_argument("reps", nargs="?", type=int, default=3)
+    ap.add_argument("scenes", nargs="?", type=int, default=256)
+    a = ap.parse_args(argv)
+    with Batch(gen.batch(a.scenes)) as b:
+        print("batch of %d scenes, %d samples, %d chunks" % (b.n, b.samples, b.chunks), flush=True)
+        for i in range(a.reps):
+            s = b.render() if a.mode == "batch" else b.render_serial()
+            print(a.mode, i, "%.2f ms" % s["trace_ms"], s["total_rays"], "rays", flush=True)
+
+
+if __name__ == "__main__":
+    _main()
