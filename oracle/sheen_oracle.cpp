@@ -26,7 +26,12 @@
 // cone/superellipsoid/heightfield/prism/CSG hits against analytic solutions;
 // BVH closest hit against brute force; Lambert/Blinn-Phong against the
 // analytic single-sphere value; photon hash-map gather against brute force;
-// mips and sRGB against closed forms.
+// mips and sRGB against closed forms; normal maps against the N.L closed
+// forms of a tilted shading normal (test_oracle_normal_map.py).
+//
+// Parity unpinned (checked only as GPU-vs-oracle agreement): heightfield
+// procedural fBm values, hair shading (shade_hair), spot-light gobos, media
+// and beams, and the caustic radiance term.
 
 #include "sheen_c_api.h"
 
