@@ -27,11 +27,13 @@
 // BVH closest hit against brute force; Lambert/Blinn-Phong against the
 // analytic single-sphere value; photon hash-map gather against brute force;
 // mips and sRGB against closed forms; normal maps against the N.L closed
-// forms of a tilted shading normal (test_oracle_normal_map.py).
+// forms of a tilted shading normal (test_oracle_normal_map.py); spot-light
+// gobos against the lit/unlit halves of a half-black gobo
+// (test_oracle_gobo.py).
 //
 // Parity unpinned (checked only as GPU-vs-oracle agreement): heightfield
-// procedural fBm values, hair shading (shade_hair), spot-light gobos, media
-// and beams, and the caustic radiance term.
+// procedural fBm values, hair shading (shade_hair), media and beams, and
+// the caustic radiance term.
 
 #include "sheen_c_api.h"
 
