@@ -1,0 +1,88 @@
+"""Hair shading's diffuse and backlight terms pinned by closed forms
+(reference shading.h:112-152, shade_hair).
+
+One large floor triangle whose first edge runs along +x gives hits with
+tangent T = +x and normal +y; a camera looks straight down at it.  With the
+specular lobes off (hair_spec_r = hair_spec_t = specular = 0), a sun along
+L adds albedo * radiance * 0.75 * (sin(T, L) + backlight * max(-T.L, 0)^2)
+to the ambient term, against albedo * radiance * max(N.L, 0) for the
+Lambert material.  So, with D the Lambert direct term under an overhead sun:
+
+* overhead sun (T.L = 0): 0.75 D;
+* sun along (0, 1, 1) (still T.L = 0): the same image;
+* sun along (1, 1, 0): 0.75 D / sqrt 2;
+* sun along (-1, 1, 0): 0.75 D (1 / sqrt 2 + backlight / 2).
+
+A tangent taken from the wrong edge, a sin/cos mix-up or a backlight term
+on the wrong side fails one of these.  The two shifted specular lobes
+depend on the per-pixel view direction and stay unpinned.  The GPU path
+is checked against the oracle with the lobes on.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from sheen_amd import contract as K
+from util import PyScene, f3, material
+
+W, H, AA = 48, 32, 2
+ALBEDO, AMB, RAD, BACK = (0.8, 0.6, 0.4), 0.1, 2.0, 0.4
+
+
+def _scene(hair=True, sun=(0.0, 1.0, 0.0), intensity=RAD, lobes=False):
+    ps = PyScene()
+    m = material(color=ALBEDO, spec=0.0, amb=AMB)
+    if hair:
+        m.anisotropy, m.hair_backlight, m.hair_shift = 1.0, BACK, 0.1
+        m.hair_spec_r = m.hair_spec_t = 0.0
+        if lobes:
+            m.hair_spec_r, m.hair_exp_r, m.hair_spec_t, m.hair_exp_t = 0.3, 40.0, 0.2, 12.0
+    ps.set("materials", K.Material, [m], "num_materials")
+    t = K.Triangle()
+    a = 50.0
+    t.v0, t.v1, t.v2 = f3(-a, 0, a), f3(a, 0, a), f3(-a, 0, -3 * a)  # first edge along +x
+    t.n0 = t.n1 = t.n2 = f3(0, 0, 0)
+    t.material_id = 0
+    ps.set("triangles", K.Triangle, [t], "num_triangles")
+    cam = ps.s.camera
+    cam.position, cam.forward, cam.right, cam.up = f3(0, 3, 0), f3(0, -1, 0), f3(1, 0, 0), f3(0, 0, -1)
+    cam.focal_length, cam.sensor_width = 50.0, 36.0
+    L = K.Light()
+    d = np.asarray(sun, np.float64)
+    d = d / np.linalg.norm(d)
+    L.type, L.direction, L.color, L.intensity = 0, f3(*(-d)), f3(1, 1, 1), float(intensity)
+    L.angle, L.flags, L.texture_id = 0.0, 1, -1
+    ps.set("lights", K.Light, [L], "num_lights")
+    return ps
+
+
+def _oracle(ps):
+    from oracle import Oracle
+    out = Oracle(ps.ptr, threads=min(8, os.cpu_count() or 1)).render(W, H, AA, hits=True)
+    assert (out["kind"] >= 0).all()
+    return out["image"][..., :3].astype(np.float64)
+
+
+def test_hair_diffuse_closed_forms():
+    dark = _oracle(_scene(intensity=0.0))
+    np.testing.assert_allclose(dark, np.broadcast_to(np.array(ALBEDO) * AMB, dark.shape), rtol=1e-6)
+    D = _oracle(_scene(hair=False)) - dark
+    np.testing.assert_allclose(D, np.broadcast_to(np.array(ALBEDO) * RAD, D.shape), rtol=1e-5)
+    over = _oracle(_scene())
+    np.testing.assert_allclose(over - dark, 0.75 * D, rtol=1e-5)
+    np.testing.assert_allclose(_oracle(_scene(sun=(0, 1, 1))), over, rtol=1e-5)
+    np.testing.assert_allclose(_oracle(_scene(sun=(1, 1, 0))) - dark, 0.75 * D / math.sqrt(2.0), rtol=1e-5)
+    np.testing.assert_allclose(_oracle(_scene(sun=(-1, 1, 0))) - dark,
+                               0.75 * D * (1.0 / math.sqrt(2.0) + BACK / 2.0), rtol=1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sun", [(0, 1, 1), (-1, 1, 0.3)])
+def test_hair_gpu_matches_oracle(sun):
+    from sheen_amd import sheen
+    ps = _scene(sun=sun, lobes=True)
+    with sheen.Context(ps.ptr) as ctx:
+        img, _ = ctx.render(W, H, AA)
+    np.testing.assert_allclose(img[..., :3], _oracle(ps), rtol=1e-4, atol=1e-6)
