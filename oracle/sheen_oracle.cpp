@@ -30,11 +30,12 @@
 // forms of a tilted shading normal (test_oracle_normal_map.py); spot-light
 // gobos against the lit/unlit halves of a half-black gobo
 // (test_oracle_gobo.py); shade_hair's diffuse and backlight terms against
-// their sin(T, L) closed forms (test_oracle_hair.py).
+// their sin(T, L) closed forms (test_oracle_hair.py); media_march against
+// the geometric series of a uniformly lit box (test_oracle_media.py).
 //
 // Parity unpinned (checked only as GPU-vs-oracle agreement): heightfield
 // procedural fBm values, the two shifted specular lobes of shade_hair,
-// media and beams, and the caustic radiance term.
+// light beams, and the caustic radiance term.
 
 #include "sheen_c_api.h"
 
