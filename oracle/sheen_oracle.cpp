@@ -29,15 +29,14 @@
 // mips and sRGB against closed forms; normal maps against the N.L closed
 // forms of a tilted shading normal (test_oracle_normal_map.py); spot-light
 // gobos against the lit/unlit halves of a half-black gobo
-// (test_oracle_gobo.py); shade_hair's diffuse and backlight terms against
-// their sin(T, L) closed forms (test_oracle_hair.py); media_march against
+// (test_oracle_gobo.py); shade_hair against its sin(T, L) and shifted-lobe
+// closed forms (test_oracle_hair.py); media_march against
 // the geometric series of a uniformly lit box (test_oracle_media.py);
 // light beams against their midpoint sum along a spot's axis
 // (test_oracle_beams.py).
 //
 // Parity unpinned (checked only as GPU-vs-oracle agreement): heightfield
-// procedural fBm values, the two shifted specular lobes of shade_hair, and
-// the caustic radiance term.
+// procedural fBm values and the caustic radiance term.
 
 #include "sheen_c_api.h"
 

@@ -14,9 +14,13 @@ Lambert material.  So, with D the Lambert direct term under an overhead sun:
 * sun along (-1, 1, 0): 0.75 D (1 / sqrt 2 + backlight / 2).
 
 A tangent taken from the wrong edge, a sin/cos mix-up or a backlight term
-on the wrong side fails one of these.  The two shifted specular lobes
-depend on the per-pixel view direction and stay unpinned.  The GPU path
-is checked against the oracle with the lobes on.
+on the wrong side fails one of these.  The two specular lobes are pinned
+with a camera so narrow that the view direction is +y within 5e-6: with
+H = normalize(L + V), the lobes add radiance * w * (1 - (T' . H)^2)^(e/2)
+for T' = normalize(T + shift N) (weight w_r, exponent e_r) and
+normalize(T - shift N) (w_t, e_t); a shift of the wrong sign, swapped
+lobes or a missing square root fails it.  The GPU path is checked against
+the oracle with the lobes on.
 """
 import math
 import os
@@ -31,7 +35,7 @@ W, H, AA = 48, 32, 2
 ALBEDO, AMB, RAD, BACK = (0.8, 0.6, 0.4), 0.1, 2.0, 0.4
 
 
-def _scene(hair=True, sun=(0.0, 1.0, 0.0), intensity=RAD, lobes=False):
+def _scene(hair=True, sun=(0.0, 1.0, 0.0), intensity=RAD, lobes=False, focal=50.0):
     ps = PyScene()
     m = material(color=ALBEDO, spec=0.0, amb=AMB)
     if hair:
@@ -48,7 +52,7 @@ def _scene(hair=True, sun=(0.0, 1.0, 0.0), intensity=RAD, lobes=False):
     ps.set("triangles", K.Triangle, [t], "num_triangles")
     cam = ps.s.camera
     cam.position, cam.forward, cam.right, cam.up = f3(0, 3, 0), f3(0, -1, 0), f3(1, 0, 0), f3(0, 0, -1)
-    cam.focal_length, cam.sensor_width = 50.0, 36.0
+    cam.focal_length, cam.sensor_width = focal, 36.0
     L = K.Light()
     d = np.asarray(sun, np.float64)
     d = d / np.linalg.norm(d)
@@ -76,6 +80,25 @@ def test_hair_diffuse_closed_forms():
     np.testing.assert_allclose(_oracle(_scene(sun=(1, 1, 0))) - dark, 0.75 * D / math.sqrt(2.0), rtol=1e-5)
     np.testing.assert_allclose(_oracle(_scene(sun=(-1, 1, 0))) - dark,
                                0.75 * D * (1.0 / math.sqrt(2.0) + BACK / 2.0), rtol=1e-5)
+
+
+def test_hair_specular_lobes_closed_form():
+    sun = np.array([1.0, 1.0, 0.0]) / math.sqrt(2.0)
+    focal = 5.0e6
+    dark = _oracle(_scene(intensity=0.0, focal=focal))
+    lobes = _oracle(_scene(sun=sun, lobes=True, focal=focal)) - dark
+    V = np.array([0.0, 1.0, 0.0])
+    Hv = (sun + V) / np.linalg.norm(sun + V)
+    T, N, shift = np.array([1.0, 0, 0]), np.array([0, 1.0, 0]), 0.1
+
+    def lobe(w, e, sign):
+        Tp = T + sign * shift * N
+        c = Tp.dot(Hv) / np.linalg.norm(Tp)
+        return w * math.sqrt(max(1.0 - c * c, 0.0)) ** max(e, 1.0)
+
+    spec = RAD * (lobe(0.3, 40.0, +1) + lobe(0.2, 12.0, -1))
+    diffuse = np.array(ALBEDO) * RAD * 0.75 / math.sqrt(2.0)
+    np.testing.assert_allclose(lobes, np.broadcast_to(diffuse + spec, lobes.shape), rtol=2e-4)
 
 
 @pytest.mark.gpu
