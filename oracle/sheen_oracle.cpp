@@ -33,10 +33,11 @@
 // closed forms (test_oracle_hair.py); media_march against
 // the geometric series of a uniformly lit box (test_oracle_media.py);
 // light beams against their midpoint sum along a spot's axis
-// (test_oracle_beams.py).
+// (test_oracle_beams.py); the caustic radiance term against a brute-force
+// gather over the read-back photon map (test_oracle_caustics.py).
 //
 // Parity unpinned (checked only as GPU-vs-oracle agreement): heightfield
-// procedural fBm values and the caustic radiance term.
+// procedural fBm values.
 
 #include "sheen_c_api.h"
 
