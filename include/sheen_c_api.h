@@ -92,8 +92,12 @@ typedef struct SheenScene {
 int sheen_init(int device_id, int cuda_arch);
 
 /* Replaces reference sheen_c_api.h:94 / api/sheen_c_api.cu:182-185.
- * Synchronises and releases every device buffer the library holds; call
- * sheen_init again before the next render. */
+ * Synchronises the device.  The reference also resets the device
+ * (cudaDeviceReset), which would free the allocations of every other HIP
+ * user in the process; here sheen_render_scene keeps no device buffer after
+ * it returns and contexts free their own in sheen_context_destroy, so there
+ * is nothing left to release.  sheen_init may be called again but is not
+ * required before the next render. */
 void sheen_cleanup(void);
 
 /* Replaces reference sheen_c_api.h:99-104 / api/sheen_c_api.cu:187-361.
@@ -256,10 +260,16 @@ int sheen_batch_info(SheenBatch* batch, int* scenes, long long* samples, int* ch
  * materials, caustics, media, beams or primitives other than spheres,
  * triangles and planes are refused (-1, reason in sheen_get_last_error).
  * `stream` is a hipStream_t (NULL = the null stream); work is ordered after
- * earlier work on it and later work on it waits for the results.  The
- * context's own materials, lights and textures are restored after each call.
+ * earlier work on it and later work on it waits for the results (an event
+ * recorded on it and waited on by the library's stream, and the reverse; if
+ * either step fails the host synchronises the earlier stream instead, see
+ * sheen_context_stream_fallbacks).  The context's own materials, lights and
+ * textures are restored after each call.
  */
 int sheen_adjoint_sizes(SheenContext* ctx, int* materials, int* lights, long long* texels);
+/* Number of caller-stream orderings of this context that fell back to host
+ * synchronisation (0 when every event record and wait succeeded). */
+int sheen_context_stream_fallbacks(const SheenContext* ctx);
 int sheen_adjoint_get_params(SheenContext* ctx, float* albedo, float* emission, float* intensity, float* texels,
                              void* stream);
 int sheen_adjoint_forward(SheenContext* ctx, int width, int height, int aa_samples, const float* albedo,

@@ -11,10 +11,14 @@ fixed).  Parameters (float32 CUDA tensors):
     intensity  (lights,)        Light.intensity
     texels     (texels,)        level-0 texels of every texture, RGB / 255
 
-torch supplies device memory and the stream; the arithmetic is in
-`sheen_amd/csrc/dev_adjoint.h`.  torch ships its own HIP runtime next to the
-system ROCm one this library links: import this module (or torch, and call
-torch.cuda.init()) before any sheen context is created in the process.
+torch supplies device memory and the stream (its current stream, often the
+null stream); the library orders its own stream after that stream on entry
+and that stream after its own on return, so torch kernels that write the
+parameters before a call and read the outputs after it need no
+synchronisation.  torch ships its own HIP runtime (soname libamdhip64.so.7,
+the same as the system ROCm one this library links): importing torch first
+makes the dynamic loader resolve the library's HIP calls to torch's runtime,
+so there is one runtime and torch's stream handles are valid here.
 """
 import ctypes as C
 
@@ -84,6 +88,10 @@ class DiffRenderer:
 
     def render(self, albedo, emission, intensity, texels):
         return _Render.apply(self, self.width, self.height, self.aa, albedo, emission, intensity, texels)
+
+    def stream_fallbacks(self):
+        """Caller-stream orderings that fell back to host synchronisation."""
+        return self._lib.sheen_context_stream_fallbacks(self._c)
 
     def set_param(self, name, value):
         native.check(self._lib.sheen_context_set_param(self._c, name.encode(), int(value)))

@@ -4,6 +4,8 @@ libsheen_host.so   host C++: scene generators, wide-BVH builder, PNG/sRGB.
 libsheen_c_api.so  HIP kernels + the C ABI of include/sheen_c_api.h.
 _sheen*.so         CPython module `sheen_amd._sheen` (reference
                    api/sheen_py_module.cpp counterpart) over libsheen_c_api.so.
+addon/sheen.so     the same module named `sheen`, the file the Blender
+                   add-on imports from its folder.
 """
 import os
 import subprocess
@@ -14,6 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INC = os.path.join(ROOT, "include")
 OBJ = os.path.join(HERE, "_obj")
+ADDON = os.path.join(HERE, "addon")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SHEEN_OFFLOAD_ARCH", "gfx950")
 
@@ -62,13 +65,22 @@ def build(force=True):
     capi = os.path.join(HERE, "libsheen_c_api.so")
     host_objs = [os.path.join(OBJ, s.replace(".cpp", ".o")) for s in HOST_SRCS]
     _run([HIPCC, "-shared", "--offload-arch=" + ARCH, "-o", capi] + hip_objs + host_objs + ["-lz"])
-    # CPython module
+    # CPython module, twice from one source: `sheen_amd._sheen` for the
+    # package, and `sheen` as the file sheen.so in addon/, the name and file
+    # the Blender add-on imports from its own folder (reference
+    # CMakeLists.txt:170-177, engine.py:29,61); its rpath finds
+    # libsheen_c_api.so beside it (an add-on install) or one folder up (here).
     pyinc = sysconfig.get_paths()["include"]
     ext = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
     mod = os.path.join(HERE, "_sheen" + ext)
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", INC, "-I", pyinc, os.path.join(CSRC, "py_sheen.cpp"),
-          "-o", mod, "-L", HERE, "-lsheen_c_api", "-Wl,-rpath,$ORIGIN"])
-    return [host_lib, capi, mod]
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-DSHEEN_PY_MODULE=_sheen", "-I", INC, "-I", pyinc,
+          os.path.join(CSRC, "py_sheen.cpp"), "-o", mod, "-L", HERE, "-lsheen_c_api", "-Wl,-rpath,$ORIGIN"])
+    os.makedirs(ADDON, exist_ok=True)
+    addon_mod = os.path.join(ADDON, "sheen.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-DSHEEN_PY_MODULE=sheen", "-I", INC, "-I", pyinc,
+          os.path.join(CSRC, "py_sheen.cpp"), "-o", addon_mod, "-L", HERE, "-lsheen_c_api",
+          "-Wl,-rpath,$ORIGIN:$ORIGIN/.."])
+    return [host_lib, capi, mod, addon_mod]
 
 
 if __name__ == "__main__":

@@ -102,6 +102,7 @@ struct SheenContext {
     Material* mats_orig = nullptr;
     Light* lights_orig = nullptr;
     hipEvent_t ev_join[2] = {nullptr, nullptr};
+    int join_fallbacks = 0;  // caller-stream joins that fell back to host synchronisation
     hipEvent_t ev_frame[2] = {nullptr, nullptr};
     // second stream: the primary shading's shadow requests are traced there
     // while the secondary rays run on `stream`
@@ -906,16 +907,27 @@ static int adj_alloc(SheenContext* c) {
     return 0;
 }
 
-// Makes the render stream wait for the caller's stream (or the reverse).
-static void adj_join(SheenContext* c, hipStream_t user, bool into_render) {
-    if (!user) return;
-    if (into_render) {
-        (void)hipEventRecord(c->ev_join[0], user);
-        (void)hipStreamWaitEvent(c->stream, c->ev_join[0], 0);
-    } else {
-        (void)hipEventRecord(c->ev_join[1], c->stream);
-        (void)hipStreamWaitEvent(user, c->ev_join[1], 0);
+// Orders the render stream after the caller's stream (into_render) or the
+// caller's stream after the render stream.  The caller's stream may be the
+// null stream (user == NULL): the render stream is non-blocking, so nothing
+// orders it against the null stream unless an event does.  When the event
+// cannot be recorded or waited on, the host waits for the earlier stream
+// instead, and the context counts the fallback.  0, or -1 with the error set.
+static int adj_join(SheenContext* c, hipStream_t user, bool into_render) {
+    hipEvent_t e = c->ev_join[into_render ? 0 : 1];
+    hipStream_t from = into_render ? user : c->stream;
+    hipStream_t to = into_render ? c->stream : user;
+    hipError_t r = hipEventRecord(e, from);
+    if (r == hipSuccess) r = hipStreamWaitEvent(to, e, 0);
+    if (r == hipSuccess) return 0;
+    (void)hipGetLastError();
+    c->join_fallbacks++;
+    r = hipStreamSynchronize(from);
+    if (r != hipSuccess) {
+        sheen_set_error("stream ordering failed: %s", hipGetErrorString(r));
+        return -1;
     }
+    return 0;
 }
 
 // Parameter set -> device materials, lights and float texel chains.
@@ -956,6 +968,8 @@ static void adj_restore(SheenContext* c) {
     c->ds.tex_f_off = nullptr;
 }
 
+int sheen_context_stream_fallbacks(const SheenContext* ctx) { return ctx ? ctx->join_fallbacks : -1; }
+
 int sheen_adjoint_sizes(SheenContext* ctx, int* materials, int* lights, long long* texels) {
     if (!ctx) { sheen_set_error("null context"); return -1; }
     long long n = 0;
@@ -971,7 +985,7 @@ int sheen_adjoint_get_params(SheenContext* ctx, float* albedo, float* emission, 
     if (!ctx || !albedo || !emission || (!intensity && ctx->ds.num_lights > 0)) { sheen_set_error("null argument"); return -1; }
     SheenContext* c = ctx;
     if (adj_alloc(c) != 0) return -1;
-    adj_join(c, (hipStream_t)stream, true);
+    if (adj_join(c, (hipStream_t)stream, true) != 0) return -1;
     const int M = c->ds.num_materials, L = c->ds.num_lights;
     k_adj_get<<<grid_for(c, std::max(M, L), 8), 256, 0, c->stream>>>(c->mats_orig, M, c->lights_orig, L, albedo, emission,
                                                                       intensity);
@@ -987,8 +1001,7 @@ int sheen_adjoint_get_params(SheenContext* ctx, float* albedo, float* emission, 
         p += n0;
     }
     HIP_OR(hipGetLastError(), -1);
-    adj_join(c, (hipStream_t)stream, false);
-    return 0;
+    return adj_join(c, (hipStream_t)stream, false);
 }
 
 int sheen_adjoint_forward(SheenContext* ctx, int width, int height, int aa_samples, const float* albedo,
@@ -997,7 +1010,7 @@ int sheen_adjoint_forward(SheenContext* ctx, int width, int height, int aa_sampl
     if (!ctx || !albedo || !emission || !image) { sheen_set_error("null argument"); return -1; }
     SheenContext* c = ctx;
     if (adj_check(c) != 0 || adj_alloc(c) != 0) return -1;
-    adj_join(c, (hipStream_t)stream, true);
+    if (adj_join(c, (hipStream_t)stream, true) != 0) return -1;
     int rc = adj_set(c, albedo, emission, intensity, texels);
     if (rc == 0) rc = render_frame(c, width, height, aa_samples, nullptr, false, nullptr, nullptr, nullptr);
     if (rc == 0 && hipMemcpyAsync(image, c->frame, (size_t)width * height * sizeof(float4), hipMemcpyDeviceToDevice,
@@ -1006,7 +1019,7 @@ int sheen_adjoint_forward(SheenContext* ctx, int width, int height, int aa_sampl
         rc = -1;
     }
     adj_restore(c);
-    adj_join(c, (hipStream_t)stream, false);
+    if (adj_join(c, (hipStream_t)stream, false) != 0) rc = -1;
     return rc;
 }
 
@@ -1018,7 +1031,7 @@ int sheen_adjoint_backward(SheenContext* ctx, int width, int height, int aa_samp
     if (adj_check(c) != 0 || adj_alloc(c) != 0) return -1;
     if (c->ds.num_lights > 0 && !d_intensity) { sheen_set_error("null intensity gradient"); return -1; }
     if (c->tex_elems > 0 && !d_texels) { sheen_set_error("null texel gradient"); return -1; }
-    adj_join(c, (hipStream_t)stream, true);
+    if (adj_join(c, (hipStream_t)stream, true) != 0) return -1;
     hipStream_t s = c->stream;
     const int M = c->ds.num_materials, L = c->ds.num_lights;
     k_fill_f<<<grid_for(c, (size_t)M * 3, 8), 256, 0, s>>>(d_albedo, (size_t)M * 3, 0.0f);
@@ -1061,8 +1074,8 @@ int sheen_adjoint_backward(SheenContext* ctx, int width, int height, int aa_samp
         }
     }
     adj_restore(c);
-    adj_join(c, (hipStream_t)stream, false);
     if (rc == 0 && hipGetLastError() != hipSuccess) { sheen_set_error("adjoint launch failed"); rc = -1; }
+    if (adj_join(c, (hipStream_t)stream, false) != 0) rc = -1;
     return rc;
 }
 
@@ -1274,6 +1287,8 @@ int sheen_batch_set_param(SheenBatch* b, const char* name, int value) {
         size_t unit = 64;
         for (const FrameDesc& F : b->frames) unit = std::max(unit, (size_t)64 * F.ns);
         if (cap < unit) { sheen_set_error("chunk smaller than a tile of the batch"); return -1; }
+        // a chunk-local slot index must fit the 28 bits SecRay.meta holds below its flags
+        if (cap > 0x0FFFFFFF) { sheen_set_error("chunk above 2^28 - 1 slots"); return -1; }
         if (!ensure_bufs(b->wb, cap, b->L, 0, false, false)) return -1;
         b->chunk_slots = cap;
         batch_chunks(b, cap);
@@ -1382,7 +1397,7 @@ int sheen_render_scene(const SheenScene* scene, int width, int height, int aa_sa
     std::vector<unsigned char> rgb((size_t)width * height * 3);
     sheen_encode_srgb(rgba.data(), width, height, rgb.data());
     if (sheen_write_png(output_path, width, height, rgb.data()) != 0) {
-        sheen_set_error("png write failed: %s", output_path);
+        sheen_set_error("stbi_write_png failed");  // the reference's message (api/sheen_c_api.cu:356)
         return -1;
     }
     return 0;

@@ -77,6 +77,7 @@ def capi():
     _sig(lib, "sheen_batch_image", C.c_int, [P, C.c_int, C.c_int, P])
     _sig(lib, "sheen_batch_info", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)])
     _sig(lib, "sheen_adjoint_sizes", C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)])
+    _sig(lib, "sheen_context_stream_fallbacks", C.c_int, [P])
     _sig(lib, "sheen_adjoint_get_params", C.c_int, [P, P, P, P, P, P])
     _sig(lib, "sheen_adjoint_forward", C.c_int, [P, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P])
     _sig(lib, "sheen_adjoint_backward", C.c_int, [P, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P, P, P, P, P])
@@ -108,6 +109,8 @@ def host():
     _sig(lib, "sheen_scene_texture", C.POINTER(C.c_ubyte), [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)])
     _sig(lib, "sheen_scene_free", None, [P])
     _sig(lib, "sheen_host_bvh_check", C.c_int, [C.POINTER(contract.SheenScene), C.c_int, C.POINTER(C.c_int)])
+    _sig(lib, "sheen_host_bvh_nodes", C.c_int, [C.POINTER(contract.SheenScene), C.c_int, P, P, P, P, C.POINTER(C.c_int)])
+    _sig(lib, "sheen_host_wbvh_check", C.c_int, [P, C.c_int, P, C.c_int, P, C.c_int, P, C.POINTER(C.c_int)])
     _sig(lib, "sheen_host_csg_boxes", C.c_int, [C.POINTER(contract.SheenScene), C.c_void_p])
     _sig(lib, "sheen_host_csg_spine", C.c_int, [C.POINTER(contract.SheenScene), C.POINTER(C.c_int), C.c_void_p, C.c_void_p])
     _sig(lib, "sheen_file_load", P, [C.c_char_p])

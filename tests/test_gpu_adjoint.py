@@ -214,3 +214,60 @@ def test_non_differentiable_scene_is_refused():
         with pytest.raises(RuntimeError, match="not differentiable: material with reflection"):
             r.render(*(p[k] for k in KEYS))
         torch.cuda.synchronize()
+
+
+def _ordering_run(setup, stream=None):
+    """Parameters written by torch kernels immediately before the forward and
+    backward calls, outputs read by torch kernels immediately after, with no
+    synchronisation in between, on torch's current stream (the null stream)
+    or on a side stream."""
+    import torch
+    r, p = setup["r"], setup["p"]
+    torch.cuda.synchronize()
+    ctx = torch.cuda.stream(stream) if stream is not None else torch.cuda.stream(torch.cuda.current_stream())
+    with ctx:
+        # a long chain of small kernels writes the parameters: a render that
+        # started before them would read stale or partial values
+        q = {k: torch.zeros_like(v) for k, v in p.items()}
+        for k in KEYS:
+            for _ in range(20):
+                q[k].add_(p[k], alpha=0.05)
+            q[k].requires_grad_(True)
+        img = r.render(*(q[k] for k in KEYS))
+        out = (img * 2.0).sum()  # read by a torch kernel right away
+        t = torch.tensor(setup["target"], device=img.device)
+        ((img[..., :3] - t[..., :3]) ** 2).sum().backward()
+        grads = {k: (q[k].grad * 1.0) for k in KEYS}
+    torch.cuda.synchronize()
+    return float(out), {k: v.cpu().numpy() for k, v in grads.items()}
+
+
+def test_caller_stream_ordering_without_synchronisation(setup):
+    """The library orders its stream after the caller's on entry and the
+    caller's after its own on return (null stream and a side stream), so the
+    unsynchronised runs equal a run with full synchronisation (the image bit
+    for bit), and no ordering fell back to host synchronisation."""
+    import torch
+    r, p = setup["r"], setup["p"]
+    before = r.stream_fallbacks()
+    ref_q = {k: torch.zeros_like(v) for k, v in p.items()}
+    for k in KEYS:
+        for _ in range(20):
+            ref_q[k].add_(p[k], alpha=0.05)
+        ref_q[k].requires_grad_(True)
+    torch.cuda.synchronize()
+    img = r.render(*(ref_q[k] for k in KEYS))
+    torch.cuda.synchronize()
+    ref_out = float((img * 2.0).sum())
+    t = torch.tensor(setup["target"], device=img.device)
+    ((img[..., :3] - t[..., :3]) ** 2).sum().backward()
+    torch.cuda.synchronize()
+    ref_g = {k: ref_q[k].grad.cpu().numpy() for k in KEYS}
+    for stream in (None, torch.cuda.Stream()):
+        out, g = _ordering_run(setup, stream)
+        assert out == ref_out
+        for k in KEYS:
+            # gradient sums use float atomics: equal up to summation order
+            scale = float(np.abs(ref_g[k]).max()) if ref_g[k].size else 0.0
+            np.testing.assert_allclose(g[k], ref_g[k], rtol=1e-5, atol=1e-6 * scale, err_msg=k)
+    assert r.stream_fallbacks() == before == 0

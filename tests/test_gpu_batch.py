@@ -47,6 +47,19 @@ def test_graph_replays_match_direct_launches(batch):
             assert np.array_equal(b.image(k), ref[k]), k
 
 
+def test_chunk_beyond_the_slot_index_range_is_refused(batch):
+    """A chunk-local slot index travels in 28 bits of a secondary-ray entry:
+    larger chunks are refused before any buffer is sized for them."""
+    b = batch["b"]
+    chunks = b.chunks
+    with pytest.raises(RuntimeError, match="chunk above 2\\^28 - 1 slots"):
+        b.set_param("chunk", 1 << 28)
+    with pytest.raises(RuntimeError, match="chunk above"):
+        b.set_param("chunk", 0x7FFFFFFF)
+    assert b.chunks == chunks
+    b.render()
+
+
 def test_chunks_across_scene_boundaries(batch):
     b = batch["b"]
     b.set_param("chunk", 0)

@@ -5,6 +5,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 import zlib
 
 import numpy as np
@@ -161,3 +162,87 @@ def test_png_writer_roundtrip(tmp_path):
     raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(13, 1 + 17 * 3)
     assert (raw[:, 0] == 0).all()
     assert np.array_equal(raw[:, 1:].reshape(13, 17, 3), img)
+
+
+def _host_tree(sc, cat):
+    H = native.host()
+    info = (C.c_int * 5)()
+    assert H.sheen_host_bvh_nodes(sc.scene, cat, None, None, None, None, info) == 0
+    n_nodes, n_prims, count = info[0], info[1], info[2]
+    nodes = np.zeros((n_nodes, 128), np.uint8)
+    prims = np.zeros(n_prims, np.uint32)
+    bounds = np.zeros(6 * count, np.float32)
+    skip = np.zeros(count, np.uint8)
+    assert H.sheen_host_bvh_nodes(sc.scene, cat, native.as_ptr(nodes), native.as_ptr(prims), native.as_ptr(bounds),
+                                  native.as_ptr(skip), info) == 0
+    return nodes, prims, bounds, skip
+
+
+def _violations(nodes, prims, bounds, skip):
+    r = (C.c_int * 2)()
+    native.host().sheen_host_wbvh_check(native.as_ptr(nodes), len(nodes), native.as_ptr(prims), len(prims),
+                                        native.as_ptr(bounds), len(skip), native.as_ptr(skip), r)
+    return r[1]
+
+
+def test_wide_bvh_check_catches_planted_boxes(scenes_mod):
+    """A child box that reaches past its parent's decoded box on either side,
+    a leaf that loses a primitive, and a primitive referenced twice are each
+    counted as violations (the check itself is pinned, not only the trees)."""
+    sc = scenes_mod.generate(2, detail=24)
+    nodes, prims, bounds, skip = _host_tree(sc, 2)
+    assert len(nodes) > 8
+    assert _violations(nodes, prims, bounds, skip) == 0
+    meta = nodes[:, 24:32]
+    q = nodes[:, 32:128].view(np.uint16).reshape(-1, 6, 8)
+    org = nodes[:, 0:12].view(np.float32).reshape(-1, 3).astype(np.float64)
+    scale = 2.0 ** (nodes[:, 12:15].astype(np.float64) - 127.0)
+    qv = q.view(np.float16).astype(np.float64)
+    # an internal child whose x-max plane, moved to the top of its grid,
+    # lies well outside the box its parent decodes for it
+    plant = None
+    for parent in range(len(nodes)):
+        for slot in range(8):
+            m = int(meta[parent, slot])
+            if m == 0xFF or (m >> 5):
+                continue
+            child = int(nodes[parent, 16:20].view(np.uint32)[0]) + (m & 31)
+            parent_hi = org[parent, 0] + qv[parent, 1, slot] * scale[parent, 0]
+            top = org[child, 0] + 2048.0 * scale[child, 0]
+            for j in range(8):
+                if meta[child, j] != 0xFF and top > parent_hi + 8 * scale[child, 0]:
+                    plant = (child, j)
+                    break
+            if plant:
+                break
+        if plant:
+            break
+    assert plant is not None
+    child, j = plant
+    bad = nodes.copy()
+    qb = bad[:, 32:128].view(np.uint16).reshape(-1, 6, 8)
+    qb[child, 1, j] = np.float16(2048.0).view(np.uint16)
+    assert _violations(bad, prims, bounds, skip) > 0
+    # lower side: the child's origin moved far below its parent's box
+    bad = nodes.copy()
+    o = bad[child, 0:12].view(np.float32)
+    o[1] -= 1000.0
+    assert _violations(bad, prims, bounds, skip) > 0
+    # a primitive referenced twice and one never referenced
+    badp = prims.copy()
+    badp[1] = badp[0]
+    assert _violations(nodes, badp, bounds, skip) > 0
+
+
+def test_one_hip_runtime_when_torch_is_imported_first():
+    """torch's bundled libamdhip64.so and the system ROCm one share the soname
+    libamdhip64.so.7, so once torch is loaded the library's HIP calls bind to
+    torch's runtime: one runtime per process, and torch's stream handles mean
+    the same thing to the library (the adjoint entry points rely on it)."""
+    script = ("import torch, sheen_amd.native as n; n.capi(); "
+              "print(sorted({l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}))")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True, cwd=root, timeout=300)
+    assert r.returncode == 0, r.stderr
+    libs = eval(r.stdout.strip().splitlines()[-1])
+    assert len(libs) == 1, libs
