@@ -7,7 +7,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
-_lib = None
+PLANT_LIB_PATH = os.path.join(HERE, "liboracle_plant.so")
+_libs = {}
 
 
 class OracleStats(C.Structure):
@@ -23,13 +24,16 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
+def lib(plant=False):
+    """liboracle.so, or with plant=True liboracle_plant.so: the same oracle
+    with the planted mistakes that oracle_plant(k) switches on (test
+    self-checks only)."""
+    if plant in _libs:
+        return _libs[plant]
+    path = PLANT_LIB_PATH if plant else LIB_PATH
+    if not os.path.exists(path):
         build()
-    L = C.CDLL(LIB_PATH)
+    L = C.CDLL(path)
     P = C.c_void_p
     L.oracle_scene_create.restype = P
     L.oracle_scene_create.argtypes = [P, C.c_int]
@@ -47,6 +51,14 @@ def lib():
     L.oracle_sample_offset.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.oracle_hf_eval.restype = C.c_float
     L.oracle_hf_eval.argtypes = [P, C.c_float, C.c_float]
+    L.oracle_hf_cell_hash.restype = C.c_float
+    L.oracle_hf_cell_hash.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.oracle_hf_value_noise.restype = C.c_float
+    L.oracle_hf_value_noise.argtypes = [C.c_float, C.c_float, C.c_int]
+    L.oracle_hf_fbm.restype = C.c_float
+    L.oracle_hf_fbm.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int]
+    if plant:
+        L.oracle_plant.argtypes = [C.c_int]
     L.oracle_photon_cell_hash.restype = C.c_int
     L.oracle_photon_cell_hash.argtypes = [C.c_int] * 4
     L.oracle_build_mips.restype = C.c_int
@@ -62,7 +74,7 @@ def lib():
     L.oracle_photon_probe_brute.argtypes = [P, C.c_int, C.c_float, C.c_int, P]
     L.oracle_set_params.argtypes = [P, P, P, P, P]
     L.oracle_render_reference.argtypes = [P] + [C.c_int] * 10 + [P]
-    _lib = L
+    _libs[plant] = L
     return L
 
 
@@ -73,9 +85,9 @@ def _p(a):
 class Oracle:
     """CPU restatement of the render path over one scene (SheenScene pointer)."""
 
-    def __init__(self, scene_ptr, threads=None):
+    def __init__(self, scene_ptr, threads=None, plant=False):
         self.threads = threads or os.cpu_count() or 1
-        self._L = lib()
+        self._L = lib(plant)
         self._h = self._L.oracle_scene_create(C.cast(scene_ptr, C.c_void_p), self.threads)
         if not self._h:
             raise RuntimeError("oracle scene creation failed")

@@ -54,6 +54,57 @@
 namespace orc {
 
 // ---------------------------------------------------------------------------
+// Planted mistakes.  The pin tests check every pin twice: the oracle must
+// pass it, and the oracle with a plausible mistake planted must fail it, so
+// that a pin is shown to be able to catch such a mistake.  The mistakes are
+// compiled only into liboracle_plant.so (-DORACLE_PLANTS), where
+// oracle_plant(k) switches mistake k on; liboracle.so, the checker and CPU
+// baseline, contains none of them (PLANT(k) is the constant false there).
+// ---------------------------------------------------------------------------
+enum Plant {
+    P_NONE = 0,
+    P_POINT_DIST = 1,      // point/area/spot attenuation I/(4 pi d) instead of d^2
+    P_NO_COS = 2,          // Lambert term without n.l (only its sign kept)
+    P_TWO_PI = 3,          // attenuation over 2 pi d^2
+    P_DISK_LINEAR = 4,     // disk jitter radius u R instead of sqrt(u) R
+    P_AREA_NO_COS = 5,     // area light without its cos_l factor
+    P_AREA_SIDE = 6,       // area light lit from the other side
+    P_AREA_HALF = 7,       // area sample u in [0, 1] instead of [-1, 1]
+    P_SPOT_RAMP = 8,       // spot edge ramp over (1 - cos_half)
+    P_SPOT_HALF_ANGLE = 9, // spot cone of half the angle
+    P_SPEC_NL = 10,        // Blinn-Phong lobe on n.l instead of n.h
+    P_SPEC_NO_GUARD = 11,  // specular added on surfaces facing away from the light
+    P_ROUGH_R2 = 12,       // shininess 2/r^2 - 2
+    P_IOR_RED = 13,        // mask IOR from the R channel
+    P_TRANS_NO_MASK = 14,  // mask transmission ignored
+    P_SKY_X = 15,          // sky gradient on dir.x
+    P_SKY_SWAP = 16,       // sky zenith colour channels reversed
+    P_REFL_NO_CLAMP = 17,  // reflection weight not clamped to 1
+    P_REFL_SWAP = 18,      // reflection blend weights exchanged
+    P_ETA_FIXED = 19,      // refraction eta 1/ior on every face
+    P_TIR_SIGN = 20,       // total internal reflection about the wrong side
+    P_W_ONCE = 21,         // refraction weight taken from the first face only
+    P_SHADOW_TMAX = 22,    // glass shadow passes shorten t_max
+    P_SHADOW_4 = 23,       // four glass passes per shadow ray
+    P_CHORD_HALF = 24,     // strand chord without its factor 2
+    P_RADIUS_REV = 25,     // strand radius interpolated from the wrong end
+    P_TAU_NO_STOP = 26,    // strand optical depth summed past 6
+    P_HF_HASH = 27,        // height-field hash with another z multiplier
+    P_HF_DETAIL = 28,      // height-field detail octave seed + 777
+    P_HF_LINEAR = 29,      // value noise without the smoothstep
+    P_HF_OCT_SEED = 30,    // fBm octave seed step 113
+    P_SIGMA = 31,          // strand extinction 5 instead of 6
+    P_SHADOW_THRESH = 32,  // glass passes shadow rays from transmission 0.3
+    P_EMISSION_LIT = 33,   // emission scaled by the ambient term
+};
+#ifdef ORACLE_PLANTS
+static int g_plant = 0;
+#define PLANT(k) (g_plant == (k))
+#else
+#define PLANT(k) false
+#endif
+
+// ---------------------------------------------------------------------------
 // Vector arithmetic (reference types.h:306-329)
 // ---------------------------------------------------------------------------
 template <class R> struct V3 { R x, y, z; };
@@ -575,17 +626,18 @@ template <class R> inline R strand_segment_tau(const Ray<R>& ray, const SweepPoi
     if (t > ray.tmax) return R(0);
     V3<R> closest = w + u * s - v * t;
     R d2 = dot(closest, closest);
-    R r = R(p0.radius) + (R(p1.radius) - R(p0.radius)) * s;
+    R r = PLANT(P_RADIUS_REV) ? R(p1.radius) + (R(p0.radius) - R(p1.radius)) * s
+                              : R(p0.radius) + (R(p1.radius) - R(p0.radius)) * s;
     R r2 = r * r;
     if (d2 >= r2) return R(0);
-    R chord = R(2.0f) * std::sqrt(fmax_(r2 - d2, R(0)));
+    R chord = (PLANT(P_CHORD_HALF) ? R(1.0f) : R(2.0f)) * std::sqrt(fmax_(r2 - d2, R(0)));
     return sigma * chord;
 }
 template <class R> inline R strand_tau(const Ray<R>& ray, const SweepStrand& st, const SweepPoint* pts, R sigma) {
     R tau = 0;
     for (int i = 0; i < st.point_count - 1; i++) {
         tau += strand_segment_tau(ray, pts[st.point_offset + i], pts[st.point_offset + i + 1], sigma);
-        if (tau > R(6.0f)) break;
+        if (tau > R(6.0f) && !PLANT(P_TAU_NO_STOP)) break;
     }
     return tau;
 }
@@ -694,7 +746,8 @@ template <class R> inline bool intersect_superellipsoid(const Ray<R>& ray, const
 }
 // reference primitives/heightfield.h:13-18 (signed wrap restated in uint32)
 inline float hf_cell_hash(int xi, int zi, int seed) {
-    uint32_t h = uint32_t(xi) * 374761393u + uint32_t(zi) * 668265263u + uint32_t(seed) * 974634097u;
+    uint32_t h = uint32_t(xi) * 374761393u + uint32_t(zi) * (PLANT(P_HF_HASH) ? 668265261u : 668265263u) +
+                 uint32_t(seed) * 974634097u;
     h = (h ^ (h >> 13)) * 1274126177u;
     h = h ^ (h >> 16);
     return float(h & 0xffffffu) / 16777216.0f;
@@ -702,8 +755,10 @@ inline float hf_cell_hash(int xi, int zi, int seed) {
 template <class R> inline R hf_value_noise(R fx, R fz, int seed) {  // heightfield.h:21-36
     int x0 = (int)std::floor(fx), z0 = (int)std::floor(fz);
     R tx = fx - R(x0), tz = fz - R(z0);
-    tx = tx * tx * (R(3.0f) - R(2.0f) * tx);
-    tz = tz * tz * (R(3.0f) - R(2.0f) * tz);
+    if (!PLANT(P_HF_LINEAR)) {
+        tx = tx * tx * (R(3.0f) - R(2.0f) * tx);
+        tz = tz * tz * (R(3.0f) - R(2.0f) * tz);
+    }
     R v00 = R(hf_cell_hash(x0, z0, seed));
     R v10 = R(hf_cell_hash(x0 + 1, z0, seed));
     R v01 = R(hf_cell_hash(x0, z0 + 1, seed));
@@ -716,7 +771,7 @@ template <class R> inline R hf_fbm(R x, R z, R freq, int octaves, int seed) {  /
     R fx = x * freq, fz = z * freq;
     R amp = R(0.5f), sum = R(0), norm = R(0);
     for (int o = 0; o < octaves; o++) {
-        sum += amp * hf_value_noise(fx, fz, seed + o * 131);
+        sum += amp * hf_value_noise(fx, fz, seed + o * (PLANT(P_HF_OCT_SEED) ? 113 : 131));
         norm += amp;
         amp *= R(0.5f);
         fx *= R(2.0f); fz *= R(2.0f);
@@ -745,7 +800,7 @@ template <class R> inline R hf_eval(const HeightField& hf, R x, R z) {  // heigh
     if (lx < R(0) || lx > R(1) || lz < R(0) || lz > R(1)) return R(0);
     if (hf.image) return hf_image_height(hf, x, z);
     R t = hf_fbm(lx, lz, R(hf.freq), hf.octaves, hf.seed);
-    R detail = hf_value_noise(lx * R(7.0f), lz * R(7.0f), hf.seed + 7777) * R(0.06f);
+    R detail = hf_value_noise(lx * R(7.0f), lz * R(7.0f), hf.seed + (PLANT(P_HF_DETAIL) ? 777 : 7777)) * R(0.06f);
     t = fmin_(t + detail - R(0.03f), R(1.0f));
     return t * R(hf.height);
 }
@@ -1154,7 +1209,7 @@ template <class R> inline R strand_bvh_tau(const Scene& S, const Ray<R>& ray, R 
         Box b = bounds_of(S, K_STRAND, i);
         if (!intersect_aabb(ray, b.mn, b.mx)) continue;
         tau += strand_tau(ray, S.s.sweep_strands[i], S.s.sweep_points, sigma);
-        if (tau > R(6.0f)) return tau;
+        if (tau > R(6.0f) && !PLANT(P_TAU_NO_STOP)) return tau;
     }
     return tau;
 }
@@ -1214,15 +1269,19 @@ inline bool has_mask(const Scene& S, const Material& m) { return tex_ok(S, m.mas
 template <class R> inline R roughness_to_shininess(R r) {  // shading.h:85-89
     R rr = fmax_(r, R(0.02f));
     R r2 = rr * rr;
-    return fmin_(fmax_(R(2.0f) / (r2 * r2) - R(2.0f), R(1.0f)), R(4096.0f));
+    return fmin_(fmax_(R(2.0f) / (PLANT(P_ROUGH_R2) ? r2 : r2 * r2) - R(2.0f), R(1.0f)), R(4096.0f));
 }
 template <class R> inline R hit_transmission(const Scene& S, const Hit<R>& h, const Material& m) {  // shading.h:92-99
     if (m.transmission <= 0.0f) return R(0);
-    if (has_mask(S, m)) return R(m.transmission) * sample_texture_lod(S.tex[m.mask_id], h.uvx, h.uvy, R(0)).x;
+    if (has_mask(S, m) && !PLANT(P_TRANS_NO_MASK))
+        return R(m.transmission) * sample_texture_lod(S.tex[m.mask_id], h.uvx, h.uvy, R(0)).x;
     return R(m.transmission);
 }
 template <class R> inline R hit_ior(const Scene& S, const Hit<R>& h, const Material& m) {  // shading.h:102-108
-    if (has_mask(S, m)) return R(1.0f) + sample_texture_lod(S.tex[m.mask_id], h.uvx, h.uvy, R(0)).y;
+    if (has_mask(S, m)) {
+        V3<R> mk_ = sample_texture_lod(S.tex[m.mask_id], h.uvx, h.uvy, R(0));
+        return R(1.0f) + (PLANT(P_IOR_RED) ? mk_.x : mk_.y);
+    }
     return R(m.ior);
 }
 template <class R> inline V3<R> shade_hair(const Material& m, const ShadeIn<R>& in, V3<R> L, V3<R> rad) {  // shading.h:112-152
@@ -1253,10 +1312,11 @@ template <class R> inline V3<R> shade_direct(const Scene& S, const Material& m, 
     if (m.anisotropy > 0.0f) return shade_hair(m, in, L, rad);
     V3<R> base = albedo(S, m, in);
     R diff = fmax_(dot(in.normal, L), R(0));
+    if (PLANT(P_NO_COS)) diff = diff > R(0) ? R(1) : R(0);
     V3<R> result = base * rad * diff;
-    if (m.specular > 0.0f && diff > R(0)) {
+    if (m.specular > 0.0f && (diff > R(0) || PLANT(P_SPEC_NO_GUARD))) {
         V3<R> hv = normalize(L + in.view);
-        R spec = std::pow(fmax_(dot(in.normal, hv), R(0)), shin);
+        R spec = std::pow(fmax_(dot(in.normal, PLANT(P_SPEC_NL) ? L : hv), R(0)), shin);
         result = result + rad * R(m.specular) * spec;
     }
     return result;
@@ -1494,7 +1554,8 @@ template <class R> inline V3<R> compute_lighting(const Scene& S, const Hit<R>& r
         in.normal = normalize(n0 + t1 * nx + t2 * ny);
     }
     V3<R> base = albedo(S, mat, in);
-    V3<R> result = base * R(mat.ambient) + cv<R>(mat.emission_color) * R(mat.emission);
+    V3<R> result = base * R(mat.ambient) +
+                   cv<R>(mat.emission_color) * (R(mat.emission) * (PLANT(P_EMISSION_LIT) ? R(mat.ambient) : R(1)));
     R shininess = R(mat.shininess);
     if (has_mask(S, mat)) {
         const Tex& mt = S.tex[mat.mask_id];
@@ -1515,6 +1576,7 @@ template <class R> inline V3<R> compute_lighting(const Scene& S, const Hit<R>& r
             if (light.type == 2) {
                 R ru = R(aa_rand(ls)) * R(2.0f) - R(1.0f);
                 R rv = R(aa_rand(ls ^ 0x9e3779b9u)) * R(2.0f) - R(1.0f);
+                if (PLANT(P_AREA_HALF)) { ru = R(aa_rand(ls)); rv = R(aa_rand(ls ^ 0x9e3779b9u)); }
                 sp = cv<R>(light.position) + cv<R>(light.u_axis) * ru + cv<R>(light.v_axis) * rv;
             }
             V3<R> tl = sp - rec.point;
@@ -1526,6 +1588,7 @@ template <class R> inline V3<R> compute_lighting(const Scene& S, const Hit<R>& r
                 V3<R> t1 = normalize(cross(L, up));
                 V3<R> t2 = cross(L, t1);
                 R r = std::sqrt(R(aa_rand(ls ^ 0x85ebca6bu))) * R(light.radius);
+                if (PLANT(P_DISK_LINEAR)) r = R(aa_rand(ls ^ 0x85ebca6bu)) * R(light.radius);
                 R phi = R(6.2831853f) * R(aa_rand(ls ^ 0xc2b2ae35u));
                 sp = sp + t1 * (std::cos(phi) * r) + t2 * (std::sin(phi) * r);
                 tl = sp - rec.point;
@@ -1533,21 +1596,24 @@ template <class R> inline V3<R> compute_lighting(const Scene& S, const Hit<R>& r
                 dist = std::sqrt(dist2);
                 L = tl / dist;
             }
-            R atten = R(light.intensity) / (R(12.5663706f) * dist2);
+            R atten = R(light.intensity) / ((PLANT(P_TWO_PI) ? R(6.2831853f) : R(12.5663706f)) *
+                                            (PLANT(P_POINT_DIST) ? dist : dist2));
             V3<R> gobo = mk<R>(1, 1, 1);
             if (light.type == 2) {
                 V3<R> n = normalize(cross(cv<R>(light.u_axis), cv<R>(light.v_axis)));
                 R cos_l = dot(n, L);
+                if (PLANT(P_AREA_SIDE)) cos_l = -cos_l;
                 if (cos_l <= R(0)) continue;
-                atten *= cos_l;
+                if (!PLANT(P_AREA_NO_COS)) atten *= cos_l;
             } else if (light.type == 3) {
                 V3<R> ts = normalize(rec.point - sp);
                 R c = dot(ts, cv<R>(light.direction));
-                R cos_half = std::cos(R(light.angle));
+                R cos_half = std::cos(R(light.angle) * (PLANT(P_SPOT_HALF_ANGLE) ? R(0.5f) : R(1.0f)));
                 if (c <= cos_half) continue;
                 R blend = fmin_(fmax_(R(light.blend), R(0)), R(1));
                 R cos_soft = std::cos(R(light.angle) * (R(1.0f) - blend));
-                if (cos_soft > cos_half + R(1.0e-4f)) atten *= fmin_((c - cos_half) / (cos_soft - cos_half), R(1.0f));
+                if (cos_soft > cos_half + R(1.0e-4f))
+                    atten *= fmin_((c - cos_half) / ((PLANT(P_SPOT_RAMP) ? R(1.0f) : cos_soft) - cos_half), R(1.0f));
                 if (tex_ok(S, light.texture_id)) {
                     V3<R> axis = cv<R>(light.direction);
                     V3<R> up = (std::fabs(axis.y) < R(0.9f)) ? mk<R>(0, 1, 0) : mk<R>(1, 0, 0);
@@ -1570,11 +1636,12 @@ template <class R> inline V3<R> compute_lighting(const Scene& S, const Hit<R>& r
         shadow.tmax = t_max;
         Ray<R> sray = shadow;
         bool blocked = false;
-        for (int pass = 0; pass < 3; pass++) {  // render_kernel.cu:247-268
+        for (int pass = 0; pass < (PLANT(P_SHADOW_4) ? 4 : 3); pass++) {  // render_kernel.cu:247-268
             Hit<R> sh;
             cnt.shadow++;
             if (!scene_intersect_opaque(S, sray, sh)) { blocked = false; break; }
-            if (hit_transmission(S, sh, s.materials[sh.material_id]) > R(0.5f)) {
+            if (hit_transmission(S, sh, s.materials[sh.material_id]) > (PLANT(P_SHADOW_THRESH) ? R(0.3f) : R(0.5f))) {
+                if (PLANT(P_SHADOW_TMAX)) sray.tmax -= sh.t + R(0.002f);
                 sray.o = sh.point + sray.d * R(0.002f);
                 continue;
             }
@@ -1585,7 +1652,7 @@ template <class R> inline V3<R> compute_lighting(const Scene& S, const Hit<R>& r
         if (blocked) continue;
         R transmittance = R(1.0f);
         if (mat.anisotropy <= 0.0f) {
-            R tau = strand_bvh_tau(S, shadow, R(6.0f));
+            R tau = strand_bvh_tau(S, shadow, PLANT(P_SIGMA) ? R(5.0f) : R(6.0f));
             transmittance = std::exp(-tau);
         }
         result = result + shade_direct(S, mat, in, L, radiance, shininess) * transmittance;
@@ -1619,7 +1686,9 @@ template <class R> inline bool ray_box(const Ray<R>& ray, V3<R> c, V3<R> h, R& t
     return true;
 }
 template <class R> inline V3<R> sky_color(V3<R> d) {  // render_kernel.cu:329-334
-    R t = R(0.5f) * (d.y + R(1.0f));
+    R t = R(0.5f) * ((PLANT(P_SKY_X) ? d.x : d.y) + R(1.0f));
+    if (PLANT(P_SKY_SWAP))
+        return mk<R>((R(1.0f) - t) * R(1.0f) + t * R(1.0f), (R(1.0f) - t) * R(1.0f) + t * R(0.7f), (R(1.0f) - t) * R(1.0f) + t * R(0.5f));
     return mk<R>((R(1.0f) - t) * R(1.0f) + t * R(0.5f), (R(1.0f) - t) * R(1.0f) + t * R(0.7f), (R(1.0f) - t) * R(1.0f) + t * R(1.0f));
 }
 // reference render_kernel.cu:339-436
@@ -1786,8 +1855,8 @@ template <class R> inline V3<R> render_sample(const Scene& S, int x, int y, int 
                 V3<R> rins = media_march(S, rr, rhit ? rrec.t : R(1.0e9f), seed, rtau, cnt);
                 rcol = rcol * std::exp(-rtau) + rins;
             }
-            R w = fmin_(R(pm.reflect), R(1.0f));
-            color = color * (R(1.0f) - w) + rcol * w;
+            R w = PLANT(P_REFL_NO_CLAMP) ? R(pm.reflect) : fmin_(R(pm.reflect), R(1.0f));
+            color = PLANT(P_REFL_SWAP) ? color * w + rcol * (R(1.0f) - w) : color * (R(1.0f) - w) + rcol * w;
         }
     }
     if (hit) {  // refraction, render_kernel.cu:736-807
@@ -1803,11 +1872,12 @@ template <class R> inline V3<R> render_sample(const Scene& S, int x, int y, int 
                 const Material& sm = s.materials[gh.material_id];
                 V3<R> n = gh.normal, d = gr.d;
                 R ior = fmax_(hit_ior(S, gh, sm), R(1.0f));
-                R eta = entering ? (R(1.0f) / ior) : ior;
+                R eta = (entering || PLANT(P_ETA_FIXED)) ? (R(1.0f) / ior) : ior;
                 R cosi = fmax_(-dot(d, n), R(1.0e-4f));
                 R k = R(1.0f) - eta * eta * (R(1.0f) - cosi * cosi);
-                V3<R> td = (k < R(0)) ? normalize(d + R(2.0f) * cosi * n) : normalize(eta * d + (eta * cosi - std::sqrt(k)) * n);
-                w *= hit_transmission(S, gh, sm);
+                V3<R> td = (k < R(0)) ? normalize(d + (PLANT(P_TIR_SIGN) ? R(-2.0f) : R(2.0f)) * cosi * n)
+                                      : normalize(eta * d + (eta * cosi - std::sqrt(k)) * n);
+                if (b == 0 || !PLANT(P_W_ONCE)) w *= hit_transmission(S, gh, sm);
                 gr.o = gh.point + td * R(0.002f);
                 gr.d = td;
                 gr.tmin = R(0.001f);
@@ -2116,6 +2186,14 @@ uint32_t oracle_aa_hash(uint32_t x) { return aa_hash(x); }
 float oracle_aa_rand(uint32_t x) { return aa_rand(x); }
 void oracle_sample_offset(int s, int n, int x, int y, float* ox, float* oy) { aa_sample_offset_n(s, n, x, y, *ox, *oy); }
 float oracle_hf_eval(const HeightField* hf, float x, float z) { return hf_eval<float>(*hf, x, z); }
+float oracle_hf_cell_hash(int xi, int zi, int seed) { return hf_cell_hash(xi, zi, seed); }
+float oracle_hf_value_noise(float fx, float fz, int seed) { return hf_value_noise<float>(fx, fz, seed); }
+float oracle_hf_fbm(float x, float z, float freq, int octaves, int seed) { return hf_fbm<float>(x, z, freq, octaves, seed); }
+
+#ifdef ORACLE_PLANTS
+// Switches planted mistake k on (0: none).  Only in liboracle_plant.so.
+void oracle_plant(int k) { g_plant = k; }
+#endif
 int oracle_photon_cell_hash(int ix, int iy, int iz, int num_cells) { return photon_cell_hash(ix, iy, iz, num_cells); }
 
 // Mip chain of an RGB texture (scene.h:230-261); returns bytes written.
