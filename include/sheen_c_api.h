@@ -195,6 +195,24 @@ const float* sheen_context_frame_device(const SheenContext* ctx);
 int sheen_context_primary_hits(SheenContext* ctx, int width, int height, int aa_samples,
                                int* kind, int* index, float* t);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Renders one frame like sheen_context_render but calls no
+ * hipMalloc, hipFree or allocating library routine: every work buffer (sample
+ * state, event and shadow queues, counters) lives in one static device array
+ * of SHEEN_NOALLOC_SCRATCH_BYTES that each device running the library holds
+ * until the process ends, and the frame is rendered in as many chunks of
+ * whole 8x8 tiles as that array requires.  out_rgba_device: device memory the
+ * caller owns, float[width*height*4], row-major.  The context's device must
+ * be the current device.  The work space of one 8x8 tile must fit the array
+ * (it grows with aa_samples and the light count); larger needs are refused
+ * with an error.  Calls that overlap on one device race on the array, which
+ * keeps the last call's contents; profiling events are not recorded.
+ * Synchronous; 0 or -1. */
+#define SHEEN_NOALLOC_SCRATCH_BYTES 268435456u /* 256 MiB */
+int sheen_context_render_noalloc(SheenContext* ctx, int width, int height, int aa_samples, float* out_rgba_device,
+                                 SheenRayStats* stats);
+
 /* 1 = record hipEvents around every kernel (SheenRayStats.*_ms). */
 int sheen_context_set_profiling(SheenContext* ctx, int enable);
 /* 1 = count node fetches / primitive tests (slower; for roofline bytes). */

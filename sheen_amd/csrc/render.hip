@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -558,19 +559,18 @@ struct AdjCall {
 };
 
 template <bool FULL>
-static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, EventBufs& EB, SheenRayStats& S,
-                      bool hits_only, int* dk, int* di, float* dt, const AdjCall* adj) {
-    WorkBufs& w = *c->wbp;
+static void run_chunk(SheenContext* c, WorkBufs& w, Counters* ctr, float4* frame, const FrameDesc& F, SampleBufs& SB,
+                      EventBufs& EB, SheenRayStats& S, bool hits_only, int* dk, int* di, float* dt, const AdjCall* adj) {
     hipStream_t s = c->stream;
     const int ns = F.ns;
     OneSrc src;
     src.S = c->ds;
     src.F = F;
-        k_reset<<<1, 64, 0, s>>>(c->ctr, EB.shadow_nl, EB.runs);
+        k_reset<<<1, 64, 0, s>>>(ctr, EB.shadow_nl, EB.runs);
         {
             Timer t(c, 0);
-            if (c->counting) k_primary<true, FULL><<<grid_for(c, F.n_slots, c->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, c->ctr);
-            else k_primary<false, FULL><<<grid_for(c, F.n_slots, c->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, c->ctr);
+            if (c->counting) k_primary<true, FULL><<<grid_for(c, F.n_slots, c->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, ctr);
+            else k_primary<false, FULL><<<grid_for(c, F.n_slots, c->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, ctr);
             S.trace_launches++;
         }
         if (hits_only) {
@@ -579,13 +579,13 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
         }
         {
             Timer t(c, 4);
-            k_shade_primary<FULL><<<grid_for(c, F.n_slots, c->shade_grid, SHADE_BLOCK_OF(FULL)), SHADE_BLOCK_OF(FULL), 0, s>>>(src, SB, EB, w.sec_q[0], c->ctr);
+            k_shade_primary<FULL><<<grid_for(c, F.n_slots, c->shade_grid, SHADE_BLOCK_OF(FULL)), SHADE_BLOCK_OF(FULL), 0, s>>>(src, SB, EB, w.sec_q[0], ctr);
         }
         if (adj) {
             // replay: shadow transmittances of the primary hits, then the adjoint
             // (differentiable scenes have no secondary rays)
             const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
-            k_shadow<false, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(src, EB, c->ctr, 0);
+            k_shadow<false, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(src, EB, ctr, 0);
             S.shadow_launches++;
             k_adjoint<<<grid_for(c, F.n_slots, c->adjoint_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, EB, adj->grad, adj->G);
             return;
@@ -601,8 +601,8 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
             Timer t(c, 2, c->side);
             const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
             const int g = grid_for(c, cap, c->shadow_grid);
-            if (c->counting) k_shadow<true, FULL><<<g, TRACE_BLOCK, 0, c->side>>>(src, EB, c->ctr, 1);
-            else k_shadow<false, FULL><<<g, TRACE_BLOCK, 0, c->side>>>(src, EB, c->ctr, 1);
+            if (c->counting) k_shadow<true, FULL><<<g, TRACE_BLOCK, 0, c->side>>>(src, EB, ctr, 1);
+            else k_shadow<false, FULL><<<g, TRACE_BLOCK, 0, c->side>>>(src, EB, ctr, 1);
             S.shadow_launches++;
         }
         if (overlap) (void)hipEventRecord(c->ev_side[1], c->side);
@@ -614,34 +614,44 @@ static void run_chunk(SheenContext* c, const FrameDesc& F, SampleBufs& SB, Event
             {
                 Timer t(c, 1);
                 if (c->counting)
-                    k_trace_sec<true, FULL><<<grid_for(c, cap, c->sec_grid), TRACE_BLOCK, 0, s>>>(src, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
+                    k_trace_sec<true, FULL><<<grid_for(c, cap, c->sec_grid), TRACE_BLOCK, 0, s>>>(src, w.sec_q[in_q], in_ctr, w.sec_hits, ctr);
                 else
-                    k_trace_sec<false, FULL><<<grid_for(c, cap, c->sec_grid), TRACE_BLOCK, 0, s>>>(src, w.sec_q[in_q], in_ctr, w.sec_hits, c->ctr);
+                    k_trace_sec<false, FULL><<<grid_for(c, cap, c->sec_grid), TRACE_BLOCK, 0, s>>>(src, w.sec_q[in_q], in_ctr, w.sec_hits, ctr);
                 S.trace_launches++;
             }
             {
                 Timer t(c, 3);
                 k_shade_sec<FULL><<<grid_for(c, cap, c->shade_sec_grid, SHADE_BLOCK_OF(FULL)), SHADE_BLOCK_OF(FULL), 0, s>>>(src, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
-                                                                       w.sec_q[out_q], out_ctr, c->ctr);
+                                                                       w.sec_q[out_q], out_ctr, ctr);
             }
         }
         {
             Timer t(c, 2);
             const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
             const int g = grid_for(c, cap, c->shadow_grid);
-            if (c->counting) k_shadow<true, FULL><<<g, TRACE_BLOCK, 0, s>>>(src, EB, c->ctr, overlap ? 2 : 0);
-            else k_shadow<false, FULL><<<g, TRACE_BLOCK, 0, s>>>(src, EB, c->ctr, overlap ? 2 : 0);
+            if (c->counting) k_shadow<true, FULL><<<g, TRACE_BLOCK, 0, s>>>(src, EB, ctr, overlap ? 2 : 0);
+            else k_shadow<false, FULL><<<g, TRACE_BLOCK, 0, s>>>(src, EB, ctr, overlap ? 2 : 0);
             S.shadow_launches++;
         }
         if (overlap) (void)hipStreamWaitEvent(s, c->ev_side[1], 0);
         {
             Timer t(c, 3);
-            k_accumulate<<<grid_for(c, F.n_slots / ns, 8), 256, 0, s>>>(src, SB, EB, c->frame);
+            k_accumulate<<<grid_for(c, F.n_slots / ns, 8), 256, 0, s>>>(src, SB, EB, frame);
         }
 }
 
+// Caller-provided buffers of a frame (the no-allocation entry point): work
+// buffers carved from the static scratch array, counters, the output image,
+// and the chunk size that fits them.
+struct FixedBufs {
+    WorkBufs wb;
+    Counters* ctr = nullptr;
+    float4* frame = nullptr;
+    size_t chunk = 0;
+};
+
 static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st, bool hits_only, int* hk, int* hi,
-                        float* ht, const AdjCall* adj = nullptr) {
+                        float* ht, const AdjCall* adj = nullptr, FixedBufs* fx = nullptr) {
     if (W <= 0 || H <= 0 || aa < 1 || aa > 64) { sheen_set_error("bad render params"); return -1; }
     if (c->ds.num_materials <= 0) { sheen_set_error("no materials"); return -1; }
     const int ns = aa;
@@ -650,7 +660,8 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     const size_t unit = (size_t)ns * 64;
     const bool caustic = c->ds.pm.photons && c->ds.caustics > 0.0f;
     size_t chunk;
-    if (c->chunk_override > 0) chunk = (size_t)c->chunk_override;
+    if (fx) chunk = fx->chunk;
+    else if (c->chunk_override > 0) chunk = (size_t)c->chunk_override;
     else {
         const size_t budget = (size_t)8 << 30;  // 8 GiB of work buffers
         chunk = budget / slot_bytes(c->ds.num_lights, c->n_beam, caustic, c->ds.num_media > 0);
@@ -659,16 +670,21 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     chunk = std::max(unit, chunk / unit * unit);
     chunk = std::min(chunk, (total + unit - 1) / unit * unit);
     if (chunk > 0x0FFFFFFF) chunk = 0x0FFFFFFF / unit * unit;
-    if (!ensure_work(c, chunk)) return -1;
     const size_t npix = (size_t)W * H;
-    if (c->frame_cap < npix) {
-        if (c->frame) (void)hipFree(c->frame);
-        c->frame = nullptr;
-        c->frame_cap = 0;
-        if (!alloc_dev((void**)&c->frame, npix * sizeof(float4))) { sheen_set_error("frame alloc failed"); return -1; }
-        c->frame_cap = npix;
+    if (!fx) {
+        if (!ensure_work(c, chunk)) return -1;
+        if (c->frame_cap < npix) {
+            if (c->frame) (void)hipFree(c->frame);
+            c->frame = nullptr;
+            c->frame_cap = 0;
+            if (!alloc_dev((void**)&c->frame, npix * sizeof(float4))) { sheen_set_error("frame alloc failed"); return -1; }
+            c->frame_cap = npix;
+        }
+        c->frame_w = W; c->frame_h = H;
     }
-    c->frame_w = W; c->frame_h = H;
+    WorkBufs& w = fx ? fx->wb : *c->wbp;
+    Counters* const ctr = fx ? fx->ctr : c->ctr;
+    float4* const frame = fx ? fx->frame : c->frame;
     int *dk = nullptr, *di = nullptr;
     float* dt = nullptr;
     if (hits_only) {
@@ -679,7 +695,6 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
         }
     }
     SheenRayStats S{};
-    WorkBufs& w = *c->wbp;
     EventBufs EB;
     EB.base = w.ev_base; EB.caustic = caustic ? w.ev_caustic : nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
     EB.shadow_q = w.shadow_q; EB.shadow_nl = w.shadow_nl; EB.seg = 3u * (uint32_t)w.cap_slots;
@@ -691,8 +706,12 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     SB.media_p = c->ds.num_media ? w.media_p : nullptr; SB.media_r = c->ds.num_media ? w.media_r : nullptr;
     SB.n_beam = c->n_beam;
     hipStream_t s = c->stream;
-    Counters zero{};
-    HIP_OR(hipMemcpyAsync(c->ctr, &zero, sizeof(Counters), hipMemcpyHostToDevice, s), -1);
+    if (fx) {
+        k_zero_counters<<<1, 64, 0, s>>>(ctr);
+    } else {
+        Counters zero{};
+        HIP_OR(hipMemcpyAsync(ctr, &zero, sizeof(Counters), hipMemcpyHostToDevice, s), -1);
+    }
     c->ev_used = 0;
     c->marks.clear();
     HIP_OR(hipEventRecord(c->ev_frame[0], s), -1);
@@ -702,13 +721,13 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
         F.g0 = (uint32_t)g0;
         F.n_slots = (uint32_t)std::min(chunk, total - g0);
         F.out0 = 0;
-        if (c->full) run_chunk<true>(c, F, SB, EB, S, hits_only, dk, di, dt, adj);
-        else run_chunk<false>(c, F, SB, EB, S, hits_only, dk, di, dt, adj);
+        if (c->full) run_chunk<true>(c, w, ctr, frame, F, SB, EB, S, hits_only, dk, di, dt, adj);
+        else run_chunk<false>(c, w, ctr, frame, F, SB, EB, S, hits_only, dk, di, dt, adj);
     }
     HIP_OR(hipEventRecord(c->ev_frame[1], s), -1);
     HIP_OR(hipGetLastError(), -1);
     Counters host{};
-    HIP_OR(hipMemcpyAsync(&host, c->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s), -1);
+    HIP_OR(hipMemcpyAsync(&host, ctr, sizeof(Counters), hipMemcpyDeviceToHost, s), -1);
     HIP_OR(hipStreamSynchronize(s), -1);
     HIP_OR(hipEventElapsedTime(&S.frame_ms, c->ev_frame[0], c->ev_frame[1]), -1);
     for (const auto& m : c->marks) {
@@ -748,6 +767,75 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
 }
 
 // ---------------------------------------------------------------------------
+// Work space of sheen_context_render_noalloc: one static device array, held by
+// every device the library runs on until the process ends.  In host code the
+// name `scratch` is a host-side stand-in, not a device address; the device
+// address of each device's copy comes from hipGetSymbolAddress, taken once
+// per device (with that device current) and kept in g_scratch.  Only that
+// address is ever passed to kernels.  Calls that overlap on one device race
+// on the array, which keeps the last call's contents.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[SHEEN_NOALLOC_SCRATCH_BYTES];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch[kMaxDevices] = {};
+
+// Device address of the current device's scratch array (looked up on the
+// first call per device).  nullptr with the error set on failure.
+static unsigned char* scratch_on_current_device() {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { sheen_set_error("hipGetDevice failed: %s", hipGetErrorString(e)); return nullptr; }
+    if (dev < 0 || dev >= kMaxDevices) { sheen_set_error("device %d beyond the scratch table", dev); return nullptr; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            sheen_set_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch[dev] = p;
+    }
+    return static_cast<unsigned char*>(g_scratch[dev]);
+}
+
+// Lays the work buffers of `slots` sample slots and the counters out from
+// `base` (256-byte aligned pieces) as ensure_bufs allocates them.  Returns
+// the bytes used; with base == nullptr only measures.
+static size_t carve_work(unsigned char* base, size_t slots, int L, int n_beam, bool caustic, bool media, WorkBufs& w,
+                         Counters** ctr) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void* {
+        void* p = base ? base + off : nullptr;
+        off += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    const size_t ev = slots * 3;
+    const size_t Lq = (size_t)std::max(L, 1);
+    w = WorkBufs();
+    *ctr = (Counters*)take(sizeof(Counters));
+    w.hit_p = (int4*)take(slots * sizeof(int4));
+    w.samp = (float4*)take(slots * sizeof(float4));
+    w.ev_base = (float4*)take(ev * sizeof(float4));
+    w.contrib = (float4*)take(ev * Lq * sizeof(float4));
+    w.trans = (float*)take(ev * Lq * sizeof(float));
+    w.shadow_q = (ShadowReq*)take(ev * Lq * sizeof(ShadowReq));
+    w.shadow_nl = (uint32_t*)take(2 * Lq * sizeof(uint32_t));
+    w.sec_q[0] = (SecRay*)take(2 * slots * sizeof(SecRay));
+    w.sec_q[1] = (SecRay*)take(2 * slots * sizeof(SecRay));
+    w.sec_hits = (int4*)take(2 * slots * sizeof(int4));
+    if (caustic) w.ev_caustic = (float4*)take(ev * sizeof(float4));
+    if (n_beam > 0) w.beam_add = (float4*)take(slots * n_beam * sizeof(float4));
+    if (media) {
+        w.media_p = (float4*)take(slots * sizeof(float4));
+        w.media_r = (float4*)take(slots * sizeof(float4));
+    }
+    w.cap_slots = slots; w.cap_L = L; w.cap_beam = n_beam; w.cap_caustic = caustic; w.cap_media = media;
+    return off;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -756,6 +844,8 @@ int sheen_init(int device_id, int cuda_arch) {
     (void)cuda_arch;
     HIP_OR(hipSetDevice(device_id), -1);
     HIP_OR(hipFree(nullptr), -1);
+    // set-up of the no-allocation entry point's scratch address on this device
+    if (!scratch_on_current_device()) return -1;
     return 0;
 }
 
@@ -801,6 +891,50 @@ int sheen_context_render(SheenContext* ctx, int width, int height, int aa_sample
 }
 
 const float* sheen_context_frame_device(const SheenContext* ctx) { return ctx ? (const float*)ctx->frame : nullptr; }
+
+int sheen_context_render_noalloc(SheenContext* ctx, int width, int height, int aa_samples, float* out_rgba_device,
+                                 SheenRayStats* stats) {
+    if (!ctx || !out_rgba_device) { sheen_set_error("null argument"); return -1; }
+    if (width <= 0 || height <= 0 || aa_samples < 1 || aa_samples > 64) { sheen_set_error("bad render params"); return -1; }
+    int dev = -1;
+    HIP_OR(hipGetDevice(&dev), -1);
+    if (dev != ctx->device) {
+        sheen_set_error("the context's device (%d) is not the current device (%d)", ctx->device, dev);
+        return -1;
+    }
+    unsigned char* base = scratch_on_current_device();
+    if (!base) return -1;
+    SheenContext* c = ctx;
+    const int L = c->ds.num_lights;
+    const bool caustic = c->ds.pm.photons && c->ds.caustics > 0.0f;
+    const bool media = c->ds.num_media > 0;
+    const size_t unit = (size_t)aa_samples * 64;  // one 8x8 tile of a pixel's samples
+    FixedBufs fx;
+    // bytes grow with the slot count (plus alignment): the largest whole
+    // number of tiles that fits the array
+    const size_t fixed = carve_work(nullptr, 0, L, c->n_beam, caustic, media, fx.wb, &fx.ctr);
+    const size_t per_unit = carve_work(nullptr, unit, L, c->n_beam, caustic, media, fx.wb, &fx.ctr) - fixed;
+    size_t chunk = 0;
+    if (fixed < SHEEN_NOALLOC_SCRATCH_BYTES && per_unit > 0)
+        chunk = (SHEEN_NOALLOC_SCRATCH_BYTES - fixed) / per_unit * unit;
+    chunk = std::min(chunk, (size_t)0x0FFFFFFF / unit * unit);
+    while (chunk >= unit &&
+           carve_work(nullptr, chunk, L, c->n_beam, caustic, media, fx.wb, &fx.ctr) > SHEEN_NOALLOC_SCRATCH_BYTES)
+        chunk -= unit;
+    if (chunk < unit) {
+        sheen_set_error("one tile of %d samples with %d lights needs %zu bytes of work space, beyond the %u-byte scratch",
+                        aa_samples, L, fixed + per_unit, (unsigned)SHEEN_NOALLOC_SCRATCH_BYTES);
+        return -1;
+    }
+    carve_work(base, chunk, L, c->n_beam, caustic, media, fx.wb, &fx.ctr);
+    fx.frame = reinterpret_cast<float4*>(out_rgba_device);
+    fx.chunk = chunk;
+    const bool prof = c->profiling;
+    c->profiling = false;  // profiling events would be created on demand
+    const int rc = render_frame(c, width, height, aa_samples, stats, false, nullptr, nullptr, nullptr, nullptr, &fx);
+    c->profiling = prof;
+    return rc;
+}
 
 int sheen_context_primary_hits(SheenContext* ctx, int width, int height, int aa_samples, int* kind, int* index,
                                float* t) {

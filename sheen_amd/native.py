@@ -64,6 +64,7 @@ def capi():
     _sig(lib, "sheen_context_destroy", None, [P])
     _sig(lib, "sheen_context_render", C.c_int, [P, C.c_int, C.c_int, C.c_int, P, C.POINTER(SheenRayStats)])
     _sig(lib, "sheen_context_frame_device", P, [P])
+    _sig(lib, "sheen_context_render_noalloc", C.c_int, [P, C.c_int, C.c_int, C.c_int, P, C.POINTER(SheenRayStats)])
     _sig(lib, "sheen_context_primary_hits", C.c_int, [P, C.c_int, C.c_int, C.c_int, P, P, P])
     _sig(lib, "sheen_context_set_profiling", C.c_int, [P, C.c_int])
     _sig(lib, "sheen_context_set_counting", C.c_int, [P, C.c_int])
