@@ -38,16 +38,21 @@ class Mode:
         from oracle import Oracle, lib
         if self.plant is not None:
             lib(True).oracle_plant(int(self.plant))
-        return Oracle(ps.ptr, threads=THREADS, plant=self.plant is not None)
+        ptr = ps.ptr if hasattr(ps, "ptr") else ps.scene  # a PyScene or a loaded/generated scene
+        return Oracle(ptr, threads=THREADS, plant=self.plant is not None)
 
-    def render(self, ps, w=4, h=4, aa=1, hits=False):
+    def render(self, ps, w=4, h=4, aa=1, hits=False, raw=False):
+        """RGB as float64; raw: the oracle's float32 RGBA image; hits: the
+        oracle's whole result."""
         from oracle import lib
         try:
             out = self.oracle(ps).render(w, h, aa, hits=hits)
         finally:
             if self.plant is not None:
                 lib(True).oracle_plant(0)
-        return out if hits else out["image"][..., :3].astype(np.float64)
+        if hits:
+            return out
+        return out["image"] if raw else out["image"][..., :3].astype(np.float64)
 
     def lib(self):
         from oracle import lib
