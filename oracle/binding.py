@@ -88,6 +88,9 @@ class Oracle:
     def __init__(self, scene_ptr, threads=None, plant=False):
         self.threads = threads or os.cpu_count() or 1
         self._L = lib(plant)
+        # the oracle's scene copy points into the caller's record arrays (it
+        # copies the SheenScene struct only): keep the scene object alive
+        self._scene_ref = scene_ptr
         self._h = self._L.oracle_scene_create(C.cast(scene_ptr, C.c_void_p), self.threads)
         if not self._h:
             raise RuntimeError("oracle scene creation failed")

@@ -218,7 +218,7 @@ __global__ ADJOINT_BOUNDS void k_adjoint(DScene S, FrameDesc F, const int4* hit_
                     const float diff = fmaxf(dot(in.normal, u.Ld), 0.0f);
                     float sp = 0.0f;
                     if (mat.specular > 0.0f && diff > 0.0f)
-                        sp = powf(fmaxf(dot(in.normal, normalize(u.Ld + in.view)), 0.0f), shin);
+                        sp = pow_r(fmaxf(dot(in.normal, normalize(u.Ld + in.view)), 0.0f), shin);
                     dbase = dbase + u.rad * (light.intensity * diff * tr);
                     const v3 dc = (base * u.rad * diff + u.rad * (mat.specular * sp)) * tr;
                     dI = dot(g, dc);

@@ -25,7 +25,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_photon_emit(DScene S, int per_l
         const float u1 = rand01(sd), u2 = rand01(sd ^ 0x9e3779b9u), u3 = rand01(sd ^ 0x85ebca6bu);
         v3 pos = cv3(light.position), dir;
         if (light.type == 3) {
-            float cos_half = cosf(light.angle);
+            float cos_half = cos_r(light.angle);
             float ct = cos_half + (1.0f - cos_half) * u1;
             float stn = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
             float phi = 6.2831853f * u2;
@@ -33,7 +33,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_photon_emit(DScene S, int per_l
             v3 up = (fabsf(ax.y) < 0.9f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
             v3 t1 = normalize(cross(ax, up));
             v3 t2 = cross(ax, t1);
-            dir = normalize(ax * ct + t1 * (cosf(phi) * stn) + t2 * (sinf(phi) * stn));
+            dir = normalize(ax * ct + t1 * (cos_r(phi) * stn) + t2 * (sin_r(phi) * stn));
         } else if (light.type == 2) {
             pos = cv3(light.position) + cv3(light.u_axis) * (u1 * 2.0f - 1.0f) + cv3(light.v_axis) * (u2 * 2.0f - 1.0f);
             v3 n = normalize(cross(cv3(light.u_axis), cv3(light.v_axis)));
@@ -43,12 +43,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_photon_emit(DScene S, int per_l
             float cz = sqrtf(fmaxf(1.0f - u3, 0.0f));
             float r = sqrtf(u3);
             float phi = 6.2831853f * rand01(sd ^ 0xc2b2ae35u);
-            dir = normalize(n * cz + t1 * (cosf(phi) * r) + t2 * (sinf(phi) * r));
+            dir = normalize(n * cz + t1 * (cos_r(phi) * r) + t2 * (sin_r(phi) * r));
         } else {
             float cz = 2.0f * u1 - 1.0f;
             float stn = sqrtf(fmaxf(1.0f - cz * cz, 0.0f));
             float phi = 6.2831853f * u2;
-            dir = mk3(stn * cosf(phi), cz, stn * sinf(phi));
+            dir = mk3(stn * cos_r(phi), cz, stn * sin_r(phi));
         }
         v3 flux = cv3(light.color) * (light.intensity / (float)per_light);
         DRay ray;

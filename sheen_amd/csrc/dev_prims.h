@@ -244,8 +244,14 @@ SHD bool hit_cap(const DRay& r, v3 c, float radius, v3 axis, DHit& h) {
     return false;
 }
 // Sphere-swept polyline (reference primitives/sweep.h:13-92).  `smax` as for
-// hit_lathe: the frusta past the running best are skipped.
-SHD bool hit_strand(const DRay& r, const SweepStrand& st, const SweepPoint* pts, DHit& h, float smax) {
+// hit_lathe: the frusta past the running best are skipped.  That is exact for
+// the distance only: a cap leaves the tangent of the scratch record alone, so
+// a cap hit carries the tangent of the last frustum that reported a hit in
+// the same call, even one past the running best.  A caller that needs the
+// tangent (the shading rebuild) passes skip = false and every frustum is
+// solved.
+SHD bool hit_strand(const DRay& r, const SweepStrand& st, const SweepPoint* pts, DHit& h, float smax,
+                    bool skip = true) {
     if (st.point_count < 2) return false;
     bool any = false;
     float closest = r.tmax;
@@ -261,7 +267,7 @@ SHD bool hit_strand(const DRay& r, const SweepStrand& st, const SweepPoint* pts,
     if (hit_cap(r, cv3(p[0].position), p[0].radius, normalize(cv3(p[0].position) - cv3(p[1].position)), tmp)) { SHEEN_TAKE() }
     for (int i = 0; i < count - 1; i++) {
         if (hit_cone(r, cv3(p[i].position), p[i].radius, cv3(p[i + 1].position), p[i + 1].radius, tmp,
-                     fminf(smax, closest))) { SHEEN_TAKE() }
+                     skip ? fminf(smax, closest) : r.tmax)) { SHEEN_TAKE() }
         if (i < count - 2) {
             if (hit_cap(r, cv3(p[i + 1].position), p[i + 1].radius,
                         normalize(cv3(p[i + 2].position) - cv3(p[i + 1].position)), tmp)) { SHEEN_TAKE() }

@@ -7,7 +7,7 @@
 #include "dev_trace.h"
 
 SHD bool tex_ok(const DScene& S, int id) { return id >= 0 && id < S.num_textures && S.textures[id].pixels; }
-SHD float tex_lod(const TextureRef& t, float uv_fp) { return log2f(fmaxf(uv_fp * (float)t.width, 1.0f)); }
+SHD float tex_lod(const TextureRef& t, float uv_fp) { return log2_r(fmaxf(uv_fp * (float)t.width, 1.0f)); }
 
 // Bilinear tap of a texture's mip chain (reference integrator/shading.h:26-58):
 // nearest level to lod, v flipped, coordinates wrapped.
@@ -101,13 +101,13 @@ SHD v3 shade_hair(const Material& m, const ShadeIn& in, v3 L, v3 rad) {  // shad
     if (w_r > 0.0f) {
         v3 Tr = normalize(T + m.hair_shift * N);
         float sr = dot(Tr, H);
-        float rr = powf(sqrtf(fmaxf(1.0f - sr * sr, 0.0f)), fmaxf(e_r, 1.0f));
+        float rr = pow_r(sqrtf(fmaxf(1.0f - sr * sr, 0.0f)), fmaxf(e_r, 1.0f));
         result = result + rad * w_r * rr;
     }
     if (w_t > 0.0f) {
         v3 Tt = normalize(T - m.hair_shift * N);
         float st = dot(Tt, H);
-        float rt = powf(sqrtf(fmaxf(1.0f - st * st, 0.0f)), fmaxf(e_t, 1.0f));
+        float rt = pow_r(sqrtf(fmaxf(1.0f - st * st, 0.0f)), fmaxf(e_t, 1.0f));
         result = result + rad * w_t * rt;
     }
     return result;
@@ -119,7 +119,7 @@ SHD v3 shade_direct(const DScene& S, const Material& m, const ShadeIn& in, v3 L,
     v3 result = base * rad * diff;
     if (m.specular > 0.0f && diff > 0.0f) {
         v3 hv = normalize(L + in.view);
-        float spec = powf(fmaxf(dot(in.normal, hv), 0.0f), shin);
+        float spec = pow_r(fmaxf(dot(in.normal, hv), 0.0f), shin);
         result = result + rad * m.specular * spec;
     }
     return result;
@@ -130,9 +130,9 @@ SHD v3 sun_dir(const Light& light, uint32_t seed) {  // render_kernel.cu:13-24
     v3 up = (fabsf(base.y) < 0.9f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
     v3 t1 = normalize(cross(base, up));
     v3 t2 = cross(base, t1);
-    float r = sqrtf(rand01(seed ^ 0x68e31da4u)) * tanf(light.angle);
+    float r = sqrtf(rand01(seed ^ 0x68e31da4u)) * tan_r(light.angle);
     float phi = 6.2831853f * rand01(seed ^ 0xb5297a4du);
-    return normalize(base + t1 * (cosf(phi) * r) + t2 * (sinf(phi) * r));
+    return normalize(base + t1 * (cos_r(phi) * r) + t2 * (sin_r(phi) * r));
 }
 SHD v3 sky(v3 d) {  // render_kernel.cu:329-334
     float t = 0.5f * (d.y + 1.0f);
@@ -174,7 +174,7 @@ SHD LightSample light_sample(const DScene& S, const Light& light, float intensit
             v3 t2 = cross(Ld, t1);
             float r = sqrtf(rand01(ls ^ 0x85ebca6bu)) * light.radius;
             float phi = 6.2831853f * rand01(ls ^ 0xc2b2ae35u);
-            sp = sp + t1 * (cosf(phi) * r) + t2 * (sinf(phi) * r);
+            sp = sp + t1 * (cos_r(phi) * r) + t2 * (sin_r(phi) * r);
             tl = sp - point;
             dist2 = fmaxf(dot(tl, tl), 1.0e-4f);
             dist = sqrtf(dist2);
@@ -190,18 +190,18 @@ SHD LightSample light_sample(const DScene& S, const Light& light, float intensit
         } else if (light.type == 3) {
             v3 ts = normalize(point - sp);
             float c = dot(ts, cv3(light.direction));
-            float cos_half = cosf(light.angle);
+            float cos_half = cos_r(light.angle);
             if (c <= cos_half) skip = true;
             else {
                 float blend = fminf(fmaxf(light.blend, 0.0f), 1.0f);
-                float cos_soft = cosf(light.angle * (1.0f - blend));
+                float cos_soft = cos_r(light.angle * (1.0f - blend));
                 if (cos_soft > cos_half + 1.0e-4f) atten *= fminf((c - cos_half) / (cos_soft - cos_half), 1.0f);
                 if (tex_ok(S, light.texture_id)) {
                     v3 axis = cv3(light.direction);
                     v3 up = (fabsf(axis.y) < 0.9f) ? mk3(0, 1, 0) : mk3(1, 0, 0);
                     v3 t1 = normalize(cross(axis, up));
                     v3 t2 = cross(axis, t1);
-                    float tan_half = tanf(fmaxf(light.angle, 1.0e-3f));
+                    float tan_half = tan_r(fmaxf(light.angle, 1.0e-3f));
                     float u = 0.5f + 0.5f * dot(ts, t1) / tan_half;
                     float v = 0.5f + 0.5f * dot(ts, t2) / tan_half;
                     gobo = tex_sample(S, light.texture_id, u, v, 0.0f);

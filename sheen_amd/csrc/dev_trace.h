@@ -776,7 +776,8 @@ SHD void rebuild_hit(const DScene& S, const DRay& r, const QHit& q, DHit& h) {
             break;
         }
         case C_STRAND:
-            if (FULL) hit_strand(r, S.strands[q.idx], S.sweep_points, h, q.t);
+            // every frustum solved: a cap hit's tangent is the last frustum hit's
+            if (FULL) hit_strand(r, S.strands[q.idx], S.sweep_points, h, r.tmax, false);
             h.object_id = -(q.idx + 1);
             break;
         case C_TRI:

@@ -134,7 +134,7 @@ SHD float shadow_eval(const DScene& S, const ShadowReq& q, TStack& st, uint32_t&
         if (!through) { blocked = true; break; }
     }
     if (blocked) return 0.0f;
-    if (FULL && (q.slot & 0x80000000u)) return expf(-strands_depth(S, orig, st));
+    if (FULL && (q.slot & 0x80000000u)) return exp_r(-strands_depth(S, orig, st));
     return 1.0f;
 }
 
@@ -156,9 +156,9 @@ SHD void beams_of(const DScene& S, const DRay& ray, bool hit, float hit_t, uint3
         if (t_end > 0.0f) {
             const int STEPS = 6;
             float dt = t_end / (float)STEPS;
-            float cos_half = cosf(light.angle);
+            float cos_half = cos_r(light.angle);
             float blend = fminf(fmaxf(light.blend, 0.0f), 1.0f);
-            float cos_soft = cosf(light.angle * (1.0f - blend));
+            float cos_soft = cos_r(light.angle * (1.0f - blend));
             for (int k = 0; k < STEPS; k++) {
                 float t = ((float)k + 0.5f) * dt;
                 v3 p = ray.o + ray.d * t;
@@ -177,7 +177,7 @@ SHD void beams_of(const DScene& S, const DRay& ray, bool hit, float hit_t, uint3
                     v3 t2 = cross(axis, t1);
                     float rr = sqrtf(rand01(seed ^ (uint32_t)(uint32_t(k) * 2654435761u) ^ 0x85ebca6bu)) * light.radius;
                     float ph = 6.2831853f * rand01(seed ^ (uint32_t)(uint32_t(k) * 2246822519u) ^ 0xc2b2ae35u);
-                    target = cv3(light.position) + t1 * (cosf(ph) * rr) + t2 * (sinf(ph) * rr);
+                    target = cv3(light.position) + t1 * (cos_r(ph) * rr) + t2 * (sin_r(ph) * rr);
                 }
                 v3 to_l = target - p;
                 float ldist = sqrtf(fmaxf(dot(to_l, to_l), 1.0e-8f));
@@ -268,10 +268,10 @@ SHD float4 media_of(const DScene& S, const DRay& ray, float t_surface, uint32_t 
                     lrad = cv3(light.color) * (light.intensity / (12.5663706f * d2));
                     if (light.type == 3) {
                         float c = dot(-ldir, cv3(light.direction));
-                        float cos_half = cosf(light.angle);
+                        float cos_half = cos_r(light.angle);
                         if (c <= cos_half) continue;
                         float blend = fminf(fmaxf(light.blend, 0.0f), 1.0f);
-                        float cos_soft = cosf(light.angle * (1.0f - blend));
+                        float cos_soft = cos_r(light.angle * (1.0f - blend));
                         if (cos_soft > cos_half + 1.0e-4f) lrad = lrad * fminf((c - cos_half) / (cos_soft - cos_half), 1.0f);
                     }
                 }
@@ -283,12 +283,12 @@ SHD float4 media_of(const DScene& S, const DRay& ray, float t_surface, uint32_t 
                 radiance = radiance + lrad;
             }
             v3 scatter = cv3(med.sigma_s) * radiance * 0.07957747f;
-            in_scatter = in_scatter + scatter * (expf(-tau) * dt);
+            in_scatter = in_scatter + scatter * (exp_r(-tau) * dt);
             tau += med.sigma_t * dt;
         }
         tau_total += tau;
     }
-    return make_float4(in_scatter.x, in_scatter.y, in_scatter.z, expf(-tau_total));
+    return make_float4(in_scatter.x, in_scatter.y, in_scatter.z, exp_r(-tau_total));
 }
 
 // First/next refraction face (reference render_kernel.cu:750-765).

@@ -31,8 +31,8 @@ def _check(img, kind, idx, t, ref):
     assert np.array_equal(t.view(np.uint32), ref["t"].view(np.uint32))
     rel = np.abs(img[..., :3] - ref["image"][..., :3]) / np.maximum(np.abs(ref["image"][..., :3]), 1e-3)
     frac = float(np.mean(rel.max(-1) > 1e-4))
-    print("beyond 1e-4: %.4f" % frac)
-    assert frac <= 0.05
+    print("beyond 1e-4: %.6f, max rel %.3g" % (frac, rel.max()), flush=True)
+    assert frac == 0.0 and rel.max() <= 1e-4
     return int(np.sum(ref["kind"] == 7))  # CSG hits
 
 

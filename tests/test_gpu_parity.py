@@ -1,10 +1,12 @@
 """GPU parity of the HIP render path against the CPU oracle.
 
 Primary hits (category, index, distance) must match bit for bit.  Radiance
-must agree within 1e-4 relative per pixel; pixels whose light samples or
-shadow decisions depend on transcendental functions (sin/cos/tan/pow/exp:
-OCML on the GPU, glibc in the oracle) may flip at a shading discontinuity,
-so those scenes bound the fraction of such pixels instead of requiring none.
+must agree within 1e-4 relative per pixel (relative error
+|gpu - oracle| / max(|oracle|, 1e-3), worst channel): every case bounds both
+the fraction of pixels beyond 1e-4 and the largest relative error, and
+prints both.  Transcendentals on the shading, light-sampling, media and
+photon paths are evaluated in double and rounded once on both sides, so the
+measured bounds are zero excursions for every case.
 """
 import os
 
@@ -13,17 +15,17 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-# (config, overrides, width, height, aa, max fraction of pixels beyond 1e-4)
+# (config, overrides, width, height, aa, max fraction of pixels beyond 1e-4, max relative error)
 CASES = [
-    ("cfg1", 1, {}, 256, 256, 1, 0.0),
-    ("mesh", 2, {"detail": 40}, 192, 108, 4, 0.0),
-    ("lights", 100, {}, 160, 120, 4, 0.01),
-    ("meshes", 101, {}, 160, 120, 4, 0.0),
-    ("textures", 102, {}, 160, 120, 4, 0.001),
-    ("caustics", 103, {}, 160, 120, 2, 0.01),
-    ("shading", 5, {"detail": 2000}, 128, 128, 2, 0.0),
-    ("analytic", 104, {}, 160, 90, 4, 0.05),
-    ("features", 105, {}, 160, 120, 4, 0.05),
+    ("cfg1", 1, {}, 256, 256, 1, 0.0, 1e-4),
+    ("mesh", 2, {"detail": 40}, 192, 108, 4, 0.0, 1e-4),
+    ("lights", 100, {}, 160, 120, 4, 0.0, 1e-4),
+    ("meshes", 101, {}, 160, 120, 4, 0.0, 1e-4),
+    ("textures", 102, {}, 160, 120, 4, 0.0, 1e-4),
+    ("caustics", 103, {}, 160, 120, 2, 0.0, 1e-4),
+    ("shading", 5, {"detail": 2000}, 128, 128, 2, 0.0, 1e-4),
+    ("analytic", 104, {}, 160, 90, 4, 0.0, 1e-4),
+    ("features", 105, {}, 160, 120, 4, 0.0, 1e-4),
 ]
 
 
@@ -36,14 +38,14 @@ def renders():
     from sheen_amd import scenes, sheen
     from oracle import Oracle
     out = {}
-    for name, cfg, kw, w, h, aa, tol in CASES:
+    for name, cfg, kw, w, h, aa, max_frac, max_rel in CASES:
         print("case", name, flush=True)
         sc = scenes.generate(cfg, **kw)
         with sheen.Context(sc.scene) as ctx:
             img, st = ctx.render(w, h, aa)
             kind, idx, t = ctx.primary_hits(w, h, aa)
         ref = Oracle(sc.scene, threads=min(16, os.cpu_count() or 1)).render(w, h, aa, hits=True)
-        out[name] = dict(img=img, st=st, kind=kind, idx=idx, t=t, ref=ref, tol=tol)
+        out[name] = dict(img=img, st=st, kind=kind, idx=idx, t=t, ref=ref, max_frac=max_frac, max_rel=max_rel)
     return out
 
 
@@ -61,8 +63,9 @@ def test_radiance_within_1e4(renders, name):
     r = renders[name]
     m = _rel(r["img"][..., :3], r["ref"]["image"][..., :3]).max(-1)
     frac = float(np.mean(m > 1e-4))
-    print(name, "max rel %.3g, beyond 1e-4: %.5f" % (m.max(), frac))
-    assert frac <= r["tol"]
+    print(name, "max rel %.3g, beyond 1e-4: %.6f" % (m.max(), frac), flush=True)
+    assert frac <= r["max_frac"]
+    assert m.max() <= r["max_rel"]
 
 
 @pytest.mark.parametrize("name", ["cfg1", "mesh", "meshes", "shading"])
