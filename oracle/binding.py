@@ -55,6 +55,8 @@ def lib(plant=False):
     L.oracle_hf_cell_hash.argtypes = [C.c_int, C.c_int, C.c_int]
     L.oracle_hf_value_noise.restype = C.c_float
     L.oracle_hf_value_noise.argtypes = [C.c_float, C.c_float, C.c_int]
+    L.oracle_math.restype = C.c_float
+    L.oracle_math.argtypes = [C.c_int, C.c_float, C.c_float]
     L.oracle_hf_fbm.restype = C.c_float
     L.oracle_hf_fbm.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int]
     if plant:

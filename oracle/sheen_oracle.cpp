@@ -39,6 +39,7 @@
 // (CUDA's single-precision library) are closed; see sin_r below.
 
 #include "sheen_c_api.h"
+#include "sheen_math.h"  // the product's transcendentals: the parity boundary is shared, not restated
 
 #include <algorithm>
 #include <atomic>
@@ -126,17 +127,19 @@ template <class R> inline V3<R> normalize(V3<R> v) {
 }
 // Transcendentals of shading, light sampling, media and photons (reference
 // render_kernel.cu, shading.h; the reference calls its library's
-// single-precision functions).  For R = float: the value in double rounded
-// once to float, the correctly rounded result except in rare hard cases,
-// which is what the HIP path evaluates, so the two agree bit for bit
-// whatever maths library either side uses.  For R = double: the double
-// function.  Superellipsoid tests and normals keep powf (hit distances).
-inline float sin_r(float x) { return (float)std::sin((double)x); }
-inline float cos_r(float x) { return (float)std::cos((double)x); }
-inline float tan_r(float x) { return (float)std::tan((double)x); }
-inline float exp_r(float x) { return (float)std::exp((double)x); }
-inline float log2_r(float x) { return (float)std::log2((double)x); }
-inline float pow_r(float x, float y) { return (float)std::pow((double)x, (double)y); }
+// single-precision functions, whose last bits no other library reproduces).
+// For R = float: sheen_math.h's double series rounded once to float, the
+// correctly rounded value except in rare hard cases; the HIP path evaluates
+// the same operations, so the two agree bit for bit.  sheen_math.h is pinned
+// against the host libm in tests/test_sheen_math.py.  For R = double: the
+// double functions of the host libm.  Superellipsoid tests and normals keep
+// powf (hit distances).
+inline float sin_r(float x) { return shm::sinf_r(x); }
+inline float cos_r(float x) { return shm::cosf_r(x); }
+inline float tan_r(float x) { return shm::tanf_r(x); }
+inline float exp_r(float x) { return shm::expf_r(x); }
+inline float log2_r(float x) { return shm::log2f_r(x); }
+inline float pow_r(float x, float y) { return shm::powf_r(x, y); }
 inline double sin_r(double x) { return std::sin(x); }
 inline double cos_r(double x) { return std::cos(x); }
 inline double tan_r(double x) { return std::tan(x); }
@@ -2205,6 +2208,17 @@ float oracle_aa_rand(uint32_t x) { return aa_rand(x); }
 void oracle_sample_offset(int s, int n, int x, int y, float* ox, float* oy) { aa_sample_offset_n(s, n, x, y, *ox, *oy); }
 float oracle_hf_eval(const HeightField* hf, float x, float z) { return hf_eval<float>(*hf, x, z); }
 float oracle_hf_cell_hash(int xi, int zi, int seed) { return hf_cell_hash(xi, zi, seed); }
+// sheen_math.h's float functions (0 sin, 1 cos, 2 tan, 3 exp, 4 log2, 5 pow).
+float oracle_math(int fn, float x, float y) {
+    switch (fn) {
+        case 0: return shm::sinf_r(x);
+        case 1: return shm::cosf_r(x);
+        case 2: return shm::tanf_r(x);
+        case 3: return shm::expf_r(x);
+        case 4: return shm::log2f_r(x);
+        default: return shm::powf_r(x, y);
+    }
+}
 float oracle_hf_value_noise(float fx, float fz, int seed) { return hf_value_noise<float>(fx, fz, seed); }
 float oracle_hf_fbm(float x, float z, float freq, int octaves, int seed) { return hf_fbm<float>(x, z, freq, octaves, seed); }
 

@@ -9,6 +9,7 @@
 #include <stdint.h>
 
 #include "sheen_c_api.h"
+#include "sheen_math.h"
 #include "wbvh.h"
 
 #define SHD __device__ __forceinline__
@@ -31,20 +32,17 @@ SHHD v3 normalize(v3 v) {
     return {v.x * inv, v.y * inv, v.z * inv};
 }
 
-// Transcendentals of the shading, light-sampling and photon paths: the value
-// in double, rounded once to float.  That is the correctly rounded float
-// result except in rare hard cases, so it does not depend on which maths
-// library computes it: the oracle evaluates the same expressions on the
-// host and agrees bit for bit.  (The reference calls the single-precision
-// functions of its own library, whose last bits no other library
-// reproduces.)  Superellipsoid tests and normals keep powf: they are part of
-// the hit distance, which matches bit for bit with powf on both sides.
-SHHD float sin_r(float x) { return (float)sin((double)x); }
-SHHD float cos_r(float x) { return (float)cos((double)x); }
-SHHD float tan_r(float x) { return (float)tan((double)x); }
-SHHD float exp_r(float x) { return (float)exp((double)x); }
-SHHD float log2_r(float x) { return (float)log2((double)x); }
-SHHD float pow_r(float x, float y) { return (float)pow((double)x, (double)y); }
+// Transcendentals of the shading, light-sampling, media and photon paths
+// (sheen_math.h): evaluated in double by series and rounded once, the same
+// bits on the GPU and in the CPU oracle.  Superellipsoid tests and normals
+// keep powf: they are part of the hit distance, which matches bit for bit
+// with powf on both sides.
+SHHD float sin_r(float x) { return shm::sinf_r(x); }
+SHHD float cos_r(float x) { return shm::cosf_r(x); }
+SHHD float tan_r(float x) { return shm::tanf_r(x); }
+SHHD float exp_r(float x) { return shm::expf_r(x); }
+SHHD float log2_r(float x) { return shm::log2f_r(x); }
+SHHD float pow_r(float x, float y) { return shm::powf_r(x, y); }
 
 // Integer hash and uniform draws (reference sampling/aa.h:32-41).
 SHHD uint32_t mix32(uint32_t x) {
