@@ -20,6 +20,7 @@ Prints one JSON line (see --help).  --gpus above 1 is refused: the path runs
 on one GPU.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -32,6 +33,43 @@ METRIC = "Mrays/s, 1M-tri 1080p; batched-vs-serial scene speedup; adjoint/forwar
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 128   # one wide BVH node per fetch
 TRI_LEAF_BYTES = 48  # vertices + index of one triangle leaf slot
+# the device code the committed counter profiles were measured on
+KERNEL_SOURCES = ["render.hip", "kernels.h", "dev_trace.h", "dev_prims.h", "dev_shade.h", "dev_common.h",
+                  "sheen_math.h", "wbvh.h"]
+
+
+def kernel_sources_sha():
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "sheen_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def counter_profile(kernel, avg_launch_ms):
+    """The committed rocprofv3 counter summary of `kernel` on this workload
+    (profiles/counters_<kernel>_mesh1m.json): HBM bytes per launch from
+    FETCH_SIZE / WRITE_SIZE, L2 hit rate, VALU issue share and lane
+    utilisation, with the hash of the device sources it was measured on and
+    whether that is the code running now."""
+    path = os.path.join(ROOT, "profiles", "counters_%s_mesh1m.json" % kernel)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        c = json.load(f)
+    traffic = c.get("hbm_bytes_per_launch")
+    out = {
+        "source": os.path.relpath(path, ROOT),
+        "measured_on_sources": c.get("sources_sha"),
+        "sources_current": c.get("sources_sha") == kernel_sources_sha(),
+        "hbm_bytes_per_launch": traffic,
+        "hbm_frac": round(traffic / (avg_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+        "l2_hit": c.get("l2_hit"),
+        "valu_issue_frac": c.get("valu_issue_frac"),
+        "lane_utilisation": c.get("lane_utilisation"),
+        "limiter": c.get("limiter"),
+    }
+    return traffic, out
 
 
 def cpu_baseline(scene, width, height, aa, budget_s=20.0):
@@ -227,11 +265,7 @@ def main():
     bytes_per_launch = frame_bytes / chunks
     avg_launch_ms = ms / (chunks * args.steps)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_%s_mesh1m.json" % dom)
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    traffic, counters = counter_profile(dom, avg_launch_ms)
 
     line = {
         "metric": METRIC,
@@ -263,6 +297,9 @@ def main():
             "traffic": traffic,
             "bytes_per_launch": int(bytes_per_launch),
             "avg_launch_ms": round(avg_launch_ms, 4),
+            # what the counters say limits the kernel: "achieved" counts the
+            # algorithmic node and triangle bytes, most of which the caches serve
+            "counters": counters,
         },
         "kernel_ms_per_step": {k: round(tot[k] / args.steps, 3) for k in ("primary_ms", "trace_ms", "shadow_ms",
                                                                         "shade_ms")},

@@ -239,8 +239,11 @@ int sheen_context_set_param(SheenContext* ctx, const char* name, int value);
  * and work buffers are shared, and the frame's launches are captured once
  * into a hipGraph and replayed.  The reference renders one scene per call
  * (sheen_c_api.cu:187-361); sheen_batch_render_serial renders the same
- * scenes that way, one render per scene, for comparison.  Scenes must hold
- * spheres, triangles and planes only, without caustics.
+ * scenes that way, one render per scene, for comparison.  Any scene may be
+ * batched: when one holds primitives beyond spheres, triangles and planes,
+ * media or spot-light beams, the whole batch runs the pipeline built for
+ * every kind (slower per ray than the spheres/triangles/planes one); each
+ * scene keeps its own photon map.
  */
 typedef struct SheenBatch SheenBatch;
 SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* widths, const int* heights,

@@ -145,8 +145,10 @@ SHD bool occluded(const DScene& S, const DRay& r, TStack& st, QHit& h) {
 }
 
 // Spot-light beams of one primary sample (reference render_kernel.cu:560-652).
-SHD void beams_of(const DScene& S, const DRay& ray, bool hit, float hit_t, uint32_t seed, float4* out, TStack& st,
-                  uint32_t& passes) {
+// out: n_out entries (the largest beam count of the scenes sharing the work
+// buffers); entries past this scene's beams are written as zero.
+SHD void beams_of(const DScene& S, const DRay& ray, bool hit, float hit_t, uint32_t seed, float4* out, int n_out,
+                  TStack& st, uint32_t& passes) {
     int k_out = 0;
     for (int i = 0; i < S.num_lights; i++) {
         const Light& light = S.lights[i];
@@ -209,6 +211,7 @@ SHD void beams_of(const DScene& S, const DRay& ray, bool hit, float hit_t, uint3
         }
         out[k_out++] = make_float4(acc.x, acc.y, acc.z, 0.0f);
     }
+    for (; k_out < n_out; k_out++) out[k_out] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 SHD bool ray_box(const DRay& r, v3 c, v3 h, float& t0, float& t1) {  // render_kernel.cu:302-325
@@ -519,7 +522,7 @@ __global__ SHADE_BOUNDS(FULL) void k_shade_primary(SRC src, SampleBufs SB, Event
             }
         }
         if (FULL && valid && SB.beam_add)
-            beams_of(S, r, hit, hit ? rec.t : 0.0f, seed, SB.beam_add + (size_t)i * SB.n_beam, st, passes);
+            beams_of(S, r, hit, hit ? rec.t : 0.0f, seed, SB.beam_add + (size_t)i * SB.n_beam, SB.n_beam, st, passes);
         if (FULL && valid && SB.media_p) SB.media_p[i] = media_of(S, r, hit ? rec.t : 1.0e9f, seed, st, passes);
         const uint32_t a = block_reserve(&C->sec_n[0], want_r, want_t);
         if (want_r) sec_q[a] = rr;

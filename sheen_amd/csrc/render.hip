@@ -1242,6 +1242,11 @@ struct SheenBatch {
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     int primary_grid = 64, sec_grid = 4, shadow_grid = 128, shade_grid = 1, shade_sec_grid = 1;  // tuned on the 256-scene batch
+    // some scene holds primitives beyond spheres, triangles and planes, media
+    // or beams (the pipeline then runs the instantiations for every kind);
+    // caustics, media and the largest beam count size the shared buffers
+    bool full = false, caustic = false, media = false;
+    int n_beam = 0;
     size_t chunk_slots = 0;
     bool use_graph = true, dirty = true;
     int scene_table = SCENE_LDS;  // LDS entries of the slot -> scene search
@@ -1289,22 +1294,31 @@ static void batch_chunks(SheenBatch* b, size_t cap) {
 // Default chunk of a batch: 64 M slots within 48 GiB of work buffers (an
 // MI355X holds 288 GB; on the 256-scene batch, 64 M-slot chunks measured 2 %
 // faster than 32 M-slot ones, fewer chunk tails).
-static size_t batch_default_chunk(int L) {
-    return std::min((size_t)64 << 20, ((size_t)48 << 30) / slot_bytes(L, 0, false, false));
+static size_t batch_default_chunk(const SheenBatch* b) {
+    return std::min((size_t)64 << 20, ((size_t)48 << 30) / slot_bytes(b->L, b->n_beam, b->caustic, b->media));
+}
+static bool batch_bufs(SheenBatch* b, size_t slots) {
+    return ensure_bufs(b->wb, slots, b->L, b->n_beam, b->caustic, b->media);
 }
 
-static void batch_enqueue(SheenBatch* b) {
+template <bool FULL>
+static void batch_enqueue_t(SheenBatch* b) {
     hipStream_t s = b->stream;
     WorkBufs& w = b->wb;
     const int cu = b->num_cu;
+    constexpr int SB_ = SHADE_BLOCK_OF(FULL);
     k_zero_counters<<<1, 64, 0, s>>>(b->ctr);
     EventBufs EB;
-    EB.base = w.ev_base; EB.caustic = nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
+    EB.base = w.ev_base; EB.caustic = b->caustic ? w.ev_caustic : nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
     EB.shadow_q = w.shadow_q; EB.shadow_nl = w.shadow_nl; EB.seg = 3u * (uint32_t)w.cap_slots; EB.L = b->L;
-    EB.runs = std::max(b->L, 1);
+    EB.runs = FULL ? 1 : std::max(b->L, 1);  // k_shadow<*, true> reads one run
     EB.cap = (uint32_t)w.cap_slots;
     SampleBufs SB;
-    SB.hit_p = w.hit_p; SB.samp = w.samp; SB.beam_add = nullptr; SB.media_p = nullptr; SB.media_r = nullptr; SB.n_beam = 0;
+    SB.hit_p = w.hit_p; SB.samp = w.samp;
+    SB.beam_add = b->n_beam ? w.beam_add : nullptr;
+    SB.media_p = b->media ? w.media_p : nullptr;
+    SB.media_r = b->media ? w.media_r : nullptr;
+    SB.n_beam = b->n_beam;
     for (const auto& ch : b->chunks) {
         ManySrc src;
         src.S = b->d_scenes; src.S1 = b->d_scenes1; src.F = b->d_frames; src.base = b->d_base; src.n = b->n;
@@ -1312,18 +1326,23 @@ static void batch_enqueue(SheenBatch* b) {
         src.g0 = ch.first; src.count = ch.second;
         const size_t n = ch.second;
         k_reset<<<1, 64, 0, s>>>(b->ctr, EB.shadow_nl, EB.runs);
-        k_primary<false, false><<<grid_n(cu, n, b->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, b->ctr);
-        k_shade_primary<false><<<grid_n(cu, n, b->shade_grid, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(src, SB, EB, w.sec_q[0], b->ctr);
+        k_primary<false, FULL><<<grid_n(cu, n, b->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, b->ctr);
+        k_shade_primary<FULL><<<grid_n(cu, n, b->shade_grid, SB_), SB_, 0, s>>>(src, SB, EB, w.sec_q[0], b->ctr);
         for (int round = 0; round < 3; round++) {
             const int in_q = round & 1, out_q = (round + 1) & 1;
-            k_trace_sec<false, false><<<grid_n(cu, 2 * n, b->sec_grid), TRACE_BLOCK, 0, s>>>(src, w.sec_q[in_q], round,
-                                                                                            w.sec_hits, b->ctr);
-            k_shade_sec<false><<<grid_n(cu, 2 * n, b->shade_sec_grid, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(
+            k_trace_sec<false, FULL><<<grid_n(cu, 2 * n, b->sec_grid), TRACE_BLOCK, 0, s>>>(src, w.sec_q[in_q], round,
+                                                                                           w.sec_hits, b->ctr);
+            k_shade_sec<FULL><<<grid_n(cu, 2 * n, b->shade_sec_grid, SB_), SB_, 0, s>>>(
                 src, SB, EB, w.sec_q[in_q], round, w.sec_hits, w.sec_q[out_q], round + 1, b->ctr);
         }
-        k_shadow<false, false><<<grid_n(cu, 3 * n * (size_t)b->L, b->shadow_grid), TRACE_BLOCK, 0, s>>>(src, EB, b->ctr, 0);
+        k_shadow<false, FULL><<<grid_n(cu, 3 * n * (size_t)b->L, b->shadow_grid), TRACE_BLOCK, 0, s>>>(src, EB, b->ctr, 0);
         k_accumulate<<<grid_n(cu, n, 8), 256, 0, s>>>(src, SB, EB, b->frame);
     }
+}
+
+static void batch_enqueue(SheenBatch* b) {
+    if (b->full) batch_enqueue_t<true>(b);
+    else batch_enqueue_t<false>(b);
 }
 
 static void batch_stats(SheenRayStats* S, const Counters& h, uint64_t samples) {
@@ -1353,10 +1372,10 @@ SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* width
         SheenContext* c = context_create(scenes[k]);
         if (!c) return nullptr;
         b->ctx.push_back(c);
-        if (c->full || (c->ds.pm.photons && c->ds.caustics > 0.0f)) {
-            sheen_set_error("batch scene %d: batches take spheres, triangles and planes without caustics", k);
-            return nullptr;
-        }
+        b->full = b->full || c->full;
+        b->caustic = b->caustic || (c->ds.pm.photons && c->ds.caustics > 0.0f);
+        b->media = b->media || c->ds.num_media > 0;
+        b->n_beam = std::max(b->n_beam, c->n_beam);
         c->wbp = &b->serial_wb;
         b->num_cu = c->num_cu;
         b->L = std::max(b->L, c->ds.num_lights);
@@ -1393,10 +1412,10 @@ SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* width
     HIP_OR(hipMemcpy(b->d_scenes1, table1.data(), sizeof(DScene) * count, hipMemcpyHostToDevice), nullptr);
     HIP_OR(hipMemcpy(b->d_frames, b->frames.data(), sizeof(FrameDesc) * count, hipMemcpyHostToDevice), nullptr);
     HIP_OR(hipMemcpy(b->d_base, b->base.data(), sizeof(uint32_t) * (count + 1), hipMemcpyHostToDevice), nullptr);
-    b->chunk_slots = batch_default_chunk(b->L);
+    b->chunk_slots = batch_default_chunk(b.get());
     b->chunk_slots = std::min<size_t>(b->chunk_slots, ((size_t)slot0 + 63) / 64 * 64 + 64 * 64);
     batch_chunks(b.get(), b->chunk_slots);
-    if (!ensure_bufs(b->wb, b->chunk_slots, b->L, 0, false, false)) return nullptr;
+    if (!batch_bufs(b.get(), b->chunk_slots)) return nullptr;
     return b.release();
 }
 
@@ -1416,14 +1435,14 @@ int sheen_batch_set_param(SheenBatch* b, const char* name, int value) {
         if (value < 1 || value > SCENE_LDS) { sheen_set_error("scene_table must be 1..%d", SCENE_LDS); return -1; }
         b->scene_table = value;
     } else if (n == "chunk") {
-        size_t cap = value > 0 ? (size_t)value : batch_default_chunk(b->L);
+        size_t cap = value > 0 ? (size_t)value : batch_default_chunk(b);
         if (value == 0) cap = std::min<size_t>(cap, ((size_t)b->base[b->n] + 63) / 64 * 64 + 64 * 64);  // no larger than the batch
         size_t unit = 64;
         for (const FrameDesc& F : b->frames) unit = std::max(unit, (size_t)64 * F.ns);
         if (cap < unit) { sheen_set_error("chunk smaller than a tile of the batch"); return -1; }
         // a chunk-local slot index must fit the 28 bits SecRay.meta holds below its flags
         if (cap > 0x0FFFFFFF) { sheen_set_error("chunk above 2^28 - 1 slots"); return -1; }
-        if (!ensure_bufs(b->wb, cap, b->L, 0, false, false)) return -1;
+        if (!batch_bufs(b, cap)) return -1;
         b->chunk_slots = cap;
         batch_chunks(b, cap);
     } else { sheen_set_error("unknown parameter: %s", name); return -1; }

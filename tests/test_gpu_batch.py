@@ -101,9 +101,39 @@ def test_batched_scene_matches_oracle(batch):
     assert rel.max() <= 1e-4
 
 
-def test_unsupported_scene_is_refused():
+def test_bad_render_params_are_refused():
     from sheen_amd import scenes
     from sheen_amd.batch import Batch
-    sc = scenes.generate(104)
-    with pytest.raises(RuntimeError, match="batches take spheres, triangles and planes"):
-        Batch([sc], [(32, 32, 1)])
+    sc = scenes.generate(1)
+    with pytest.raises(RuntimeError, match="bad render params for batch scene 1"):
+        Batch([sc, sc], [(32, 32, 1), (32, 32, 65)])
+
+
+def test_batch_of_every_primitive_kind_media_beams_and_caustics():
+    """Scenes with analytic primitives, CSG, hair, media, spot beams and their
+    own photon maps batch with plain ones: every batched image equals the
+    scene's own render bit for bit, under the pipeline built for every kind,
+    with and without the captured graph, and one of them equals the oracle."""
+    from oracle import Oracle
+    from sheen_amd import scenes
+    from sheen_amd.batch import Batch
+    sc = [scenes.generate(104), scenes.generate(105), scenes.generate(103),
+          scenes.load_sheen(os.path.join(os.path.dirname(__file__), "scenes", "media.sheen")),
+          scenes.generate(4, seed=scenes.BATCH_SEED + 3, detail=3000), scenes.generate(1)]
+    sizes = [(96, 54, 2), (80, 60, 3), (64, 48, 2), (72, 56, 2), (64, 64, 4), (48, 40, 1)]
+    with Batch(sc, sizes) as b:
+        sb = b.render()
+        ss = b.render_serial()
+        for key in ("samples", "primary_rays", "shadow_rays", "reflection_rays", "refraction_rays", "total_rays"):
+            assert sb[key] == ss[key], key
+        ref = [b.image(k, serial=True) for k in range(b.n)]
+        for k in range(b.n):
+            assert np.array_equal(b.image(k), ref[k]), k
+        b.set_param("graph", 0)
+        b.render()
+        for k in range(b.n):
+            assert np.array_equal(b.image(k), ref[k]), k
+        W, H, A = sizes[1]
+        o = Oracle(sc[1].scene, threads=min(16, os.cpu_count() or 1)).render(W, H, A)["image"]
+        rel = np.abs(ref[1][..., :3] - o[..., :3]) / np.maximum(np.abs(o[..., :3]), 1e-3)
+        assert rel.max() <= 1e-4, rel.max()
