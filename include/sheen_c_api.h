@@ -150,6 +150,10 @@ typedef struct SheenRayStats {
     int trace_launches;
     int shadow_launches;
     float shade_primary_ms;          /* the primary-hit shading kernel alone */
+    /* With counting enabled: 64 x the node-visit iterations of the shadow
+     * kernel's waves; shadow_node_fetches / shadow_visit_lanes is the share
+     * of wave lanes that visit a node in the traversal loop. */
+    uint64_t shadow_visit_lanes;
 } SheenRayStats;
 
 typedef struct SheenContext SheenContext;

@@ -85,6 +85,7 @@ struct DScene {
     int count[N_BVH_CATS];
     int cat_opaque[N_BVH_CATS];            // 1 = no material of the category can pass a shadow ray
     const float4* tri_leaf;                // 3 float4 per triangle leaf slot: v0|idx, v1, v2
+    float box_k[N_BVH_CATS][3];            // per BVH and axis: max of 2|origin| + 2 Q scale over its nodes
 
     const Object* objects;
     const SweepStrand* strands;

@@ -299,6 +299,45 @@ SHD uint32_t block_reserve(uint32_t* counter, bool a, bool b) {
     return (a || b) ? base : 0xFFFFFFFFu;
 }
 
+// Block-wide reservation of up to two queue entries per thread, one of
+// class A (flag a) and one of class B (flag b), with one device atomic per
+// block: the block's entries form one contiguous range holding every class-A
+// entry first, then every class-B entry, each class in thread order.  The
+// per-(class, wave) counts are scanned by the first wave.  Every thread of
+// the block must make the call at the same point.  ra / rb: the entries
+// (0xFFFFFFFF where not asked for).
+SHD void block_reserve_ab(uint32_t* counter, bool a, bool b, uint32_t& ra, uint32_t& rb) {
+    __shared__ uint32_t s_cnt[2 * (1024 / 64)];  // up to 1024-thread blocks
+    const uint64_t ma = __ballot(a), mb = __ballot(b);
+    const int lane = (int)__lane_id();
+    const int wave = (int)(threadIdx.x >> 6);
+    const int nw = (int)(blockDim.x >> 6);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    if (lane == 0) {
+        s_cnt[wave] = (uint32_t)__popcll(ma);
+        s_cnt[nw + wave] = (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t own = lane < 2 * nw ? s_cnt[lane] : 0u;
+        uint32_t v = own;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(v, off);
+            if (lane >= off) v += t;
+        }
+        const uint32_t tot = __shfl(v, 63);
+        uint32_t base = 0;
+        if (lane == 0 && tot) base = atomicAdd(counter, tot);
+        base = __shfl(base, 0);
+        if (lane < 2 * nw) s_cnt[lane] = base + v - own;
+    }
+    __syncthreads();
+    ra = a ? s_cnt[wave] + (uint32_t)__popcll(ma & lt) : 0xFFFFFFFFu;
+    rb = b ? s_cnt[nw + wave] + (uint32_t)__popcll(mb & lt) : 0xFFFFFFFFu;
+    __syncthreads();  // s_cnt is reused by the next call
+}
+
 // Shadow-ray request: 32 bytes.
 struct ShadowReq {
     float ox, oy, oz, tmax;
@@ -321,9 +360,19 @@ struct EventBufs {
     uint32_t* shadow_nl;  // [2 runs]: requests queued per run, then their count after the primary shading
     uint32_t seg;         // queue entries per run (3 cap): run r's requests at r * seg
     int runs;             // queue runs: L (light l's requests are run l) or 1 (one queue shared by the lights)
+    int order;            // queue order (ORDER_* bits): thread order when clear
     int L;         // lights per event in contrib/trans (a batch: the most of any scene)
     uint32_t cap;  // sample slots of the work buffers (3 * cap * max(L, 1) < 2^31)
 };
+// EventBufs::order bits.  ORDER_SHADOW: a shading block's shadow requests of
+// one light queue those leaving the surface first, then those entering it
+// (dot(normal, direction) < 0: the ray crosses the object and traces
+// longer), so that a wave of the shadow kernel holds rays of one kind.
+// ORDER_SEC: a primary-shading block queues its reflection rays first, then
+// its refraction rays.  Results are written per sample slot, so the order
+// changes no image.
+#define ORDER_SHADOW 1
+#define ORDER_SEC 2
 SHD uint32_t ev_of(const EventBufs& B, uint32_t slot, uint32_t kind) { return kind * B.cap + slot; }
 SHD uint32_t ev_light(const EventBufs& B, uint32_t ev, int l) { return ev + (uint32_t)l * 3u * B.cap; }
 // sample slot of a per-light entry (a shadow request's index)
@@ -444,7 +493,15 @@ SHD void shade_event(const DScene& S, bool active, const DHit& rec, v3 view_dir,
             B.trans[ev_light(B, ev, i)] = 0.0f;
         }
         const int run = B.runs > 1 ? i : 0;
-        const uint32_t q = (uint32_t)run * B.seg + block_reserve(B.shadow_nl + run, emit, false);
+        uint32_t q;
+        if (B.order & ORDER_SHADOW) {
+            const bool into = dot(rec.normal, sd) < 0.0f;
+            uint32_t qa, qb;
+            block_reserve_ab(B.shadow_nl + run, emit && !into, emit && into, qa, qb);
+            q = (uint32_t)run * B.seg + (into ? qb : qa);
+        } else {
+            q = (uint32_t)run * B.seg + block_reserve(B.shadow_nl + run, emit, false);
+        }
         if (emit) {
             ShadowReq r;
             r.ox = so.x; r.oy = so.y; r.oz = so.z; r.tmax = stmax;

@@ -111,6 +111,7 @@ SHD void stack_set_top(TStack& st, int sp, uint2 e) { st.lds[(uint32_t)(sp - 1) 
 
 struct TraceStats {
     uint32_t nodes, prims;
+    uint32_t witers;  // node-visit iterations of the wave, counted by its lowest active lane
 };
 
 // Leaf test of one primitive reference: reference gate, then the distance.
@@ -157,15 +158,48 @@ SHD bool leaf_dist(const DScene& S, uint32_t slot, const DRay& r, v3 inv, float&
 }
 
 // State of one in-flight traversal of one category's wide BVH: the node to
-// visit next, the stack depth, and the running best (ct: acceptance bound,
-// exclusive unless a same-category tie; ci/ch: best index / found).
+// visit next, the stack depth, the running best (ct: acceptance bound,
+// exclusive unless a same-category tie; ci/ch: best index / found), and the
+// ray's box-test margins (E, fast: see trav_begin).
 struct TravState {
     uint32_t node;
     int sp;
     float ct;
     int ci;
     bool ch;
+    bool fast;
+    v3 E;
 };
+
+// Margins of the distance-space box test (trav_step) for one ray.  A plane q
+// of axis a decodes at t = q*B_a + A_a with B_a = s_a*inv_a (exact: s_a is a
+// power of two) and A_a = (o_a - ray_a)*inv_a; rounding makes this differ
+// from the exact-path value ((q*s+o) - ray)*inv by at most
+// u |inv_a| (7|o_a| + 6|ray_a| + 4 Q s_a) + 2u E_a (u = 2^-24, Q = 2048).
+// The scene's box_k[cat][a] bounds 2|o_a| + 2 Q s_a over the nodes of the
+// category's BVH, so E_a = 2^-20 |inv_a| (box_k[cat][a] + |ray_a|) (16u
+// times a bound of twice the terms) covers every node; the 2^-100 floor covers a result or
+// operand flushed from the denormal range.  Near planes move by -E_a and far
+// planes by +E_a, so the test passes every box the exact path passes.  The
+// margin is per axis: an axis the ray nearly parallels has huge |inv| and
+// planes far off in t, and must not widen the other axes.  A ray whose
+// margins or offsets could overflow (a zero direction component, huge
+// coordinates) takes the exact path (fast = false; the whole wave does).
+SHD void trav_begin(const DScene& S, int cat, const DRay& r, v3 inv, TravState& T) {
+    const float* K = S.box_k[cat];
+    const float xa = fabsf(inv.x) * (K[0] + fabsf(r.o.x));
+    const float ya = fabsf(inv.y) * (K[1] + fabsf(r.o.y));
+    const float za = fabsf(inv.z) * (K[2] + fabsf(r.o.z));
+    T.fast = xa < 0x1p120f && ya < 0x1p120f && za < 0x1p120f;  // false for NaN too
+    T.E = mk3(fmaf(xa, 0x1p-20f, 0x1p-100f), fmaf(ya, 0x1p-20f, 0x1p-100f), fmaf(za, 0x1p-20f, 0x1p-100f));
+}
+
+// Second smallest of {a, b, c} when a <= b: the running second-nearest key.
+SHD uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
 
 // CAT_MESH: the category (sphere or triangle) is chosen at run time.
 #define CAT_MESH (-1)
@@ -215,11 +249,15 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
     const float sx = __uint_as_float((h0.w & 0xFFu) << 23);
     const float sy = __uint_as_float(((h0.w >> 8) & 0xFFu) << 23);
     const float sz = __uint_as_float(((h0.w >> 16) & 0xFFu) << 23);
+    // slots 0..ni-1: internal children (node child_base + slot); ni..nc-1: leaves
+    const uint32_t nc = (h0.w >> 24) & 15u, ni = h0.w >> 28;
     const uint32_t child_base = h1.x, prim_base = h1.y;
     const uint64_t meta = (uint64_t)h1.z | ((uint64_t)h1.w << 32);
     const float bound = T.ct + T.ct * CULL_MARGIN;
-    uint32_t keys[8];
-    uint32_t leafmask = 0;
+    // hit children as slot bits; the two nearest internal children hit as
+    // keys (entry distance bits with the low 3 bits replaced by the slot;
+    // distances are >= t_min > 0 for every query that culls by them)
+    uint32_t hitbits = 0, k0 = 0xFFFFFFFFu, k1 = 0xFFFFFFFFu;
     // child plane i of a row: the fp16 pattern of an integer 0..WBVH_QMAX
     auto word = [](uint4 row, int i) -> uint32_t {
         const int w = i >> 1;
@@ -233,42 +271,28 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
     // (v_fma_mix_f32, the half taken straight from the word), product and sum
     // fused and rounded once as fmaf of the exact float
     auto gfma = [&](uint4 row, int i, float b, float a) -> float { return fmaf(g(row, i), b, a); };
-    // Distance-space decode: a plane q of axis a lies at t = q*B_a + A_a with
-    // B_a = s_a*inv_a (exact: s_a is a power of two) and A_a = (o_a - ray_a)*inv_a.
-    // Rounding makes this differ from the exact-path value ((q*s+o) - ray)*inv
-    // by less than E_a = 2^-20 (|A_a| + Q|B_a| + (|o_a| + Q s_a)|inv_a|), Q = 2048
-    // (the terms need about 7u|A| + 4uQ|B| + u(|o|+Qs)|inv|, u = 2^-24; the
-    // 2^-100 floor covers a result or operand flushed from the denormal range),
-    // so near planes offset by -E_a and far planes by +E_a bound the exact-path
-    // interval from outside and the test stays a superset.  The offset is per
-    // axis: an axis the ray nearly parallels has huge |inv| and planes far off
-    // in t, and its E must not widen the other axes.  A node whose E is not
-    // finite (a zero direction component, overflow) takes the exact path for
-    // the whole wave.
-    const float Ax = (ox - r.o.x) * inv.x, Ay = (oy - r.o.y) * inv.y, Az = (oz - r.o.z) * inv.z;
-    const float Bx = sx * inv.x, By = sy * inv.y, Bz = sz * inv.z;
-    const float Q = (float)WBVH_QMAX;
-    const float Ex = fmaf(fmaf(fmaf(Q, sx, fabsf(ox)), fabsf(inv.x), fmaf(Q, fabsf(Bx), fabsf(Ax))), 0x1p-20f, 0x1p-100f);
-    const float Ey = fmaf(fmaf(fmaf(Q, sy, fabsf(oy)), fabsf(inv.y), fmaf(Q, fabsf(By), fabsf(Ay))), 0x1p-20f, 0x1p-100f);
-    const float Ez = fmaf(fmaf(fmaf(Q, sz, fabsf(oz)), fabsf(inv.z), fmaf(Q, fabsf(Bz), fabsf(Az))), 0x1p-20f, 0x1p-100f);
-    const float inf = __builtin_huge_valf();
-    const bool finite = Ex < inf && Ey < inf && Ez < inf;  // false for NaN too
-    if (BOX_FAST && __ballot(!finite) == 0ull) {
-        const float Anx = Ax - Ex, Afx = Ax + Ex, Any = Ay - Ey, Afy = Ay + Ey, Anz = Az - Ez, Afz = Az + Ez;
+    auto take = [&](int i, float tn, float tf) {
+        const bool hit = tn <= tf;
+        hitbits |= hit ? (1u << i) : 0u;
+        const uint32_t key = (hit && (uint32_t)i < ni) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
+        k1 = umed3(k0, k1, key);
+        k0 = min(k0, key);
+    };
+    if (BOX_FAST && __ballot(!T.fast) == 0ull) {
+        // distance-space decode with the ray's margins (trav_begin)
+        const float Ax = (ox - r.o.x) * inv.x, Ay = (oy - r.o.y) * inv.y, Az = (oz - r.o.z) * inv.z;
+        const float Bx = sx * inv.x, By = sy * inv.y, Bz = sz * inv.z;
+        const float Anx = Ax - T.E.x, Afx = Ax + T.E.x, Any = Ay - T.E.y, Afy = Ay + T.E.y;
+        const float Anz = Az - T.E.z, Afz = Az + T.E.z;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
             const float tn = fmaxf(fmaxf(fmaxf(gfma(nx, i, Bx, Anx), gfma(ny, i, By, Any)), gfma(nz, i, Bz, Anz)), r.tmin);
             const float tf = fminf(fminf(fminf(gfma(fx, i, Bx, Afx), gfma(fy, i, By, Afy)), gfma(fz, i, Bz, Afz)), bound);
-            const bool hit = (m != WBVH_EMPTY) && (tn <= tf);
-            const bool leaf = (m >> 5) != 0;
-            keys[i] = (hit && !leaf) ? ((__float_as_uint(tn) & ~7u) | (m & 7u)) : 0xFFFFFFFFu;
-            if (hit && leaf) leafmask |= 1u << i;
+            take(i, tn, tf);
         }
     } else {
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            const uint32_t m = (uint32_t)(meta >> (8 * i)) & 0xFFu;
             const float lx = fmaf(g(nx, i), sx, ox), hx = fmaf(g(fx, i), sx, ox);
             const float ly = fmaf(g(ny, i), sy, oy), hy = fmaf(g(fy, i), sy, oy);
             const float lz = fmaf(g(nz, i), sz, oz), hz = fmaf(g(fz, i), sz, oz);
@@ -277,12 +301,11 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
             const float az = (lz - r.o.z) * inv.z, bz = (hz - r.o.z) * inv.z;
             const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), r.tmin));
             const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), bound));
-            const bool hit = (m != WBVH_EMPTY) && (tn <= tf);
-            const bool leaf = (m >> 5) != 0;
-            keys[i] = (hit && !leaf) ? ((__float_as_uint(tn) & ~7u) | (m & 7u)) : 0xFFFFFFFFu;
-            if (hit && leaf) leafmask |= 1u << i;
+            take(i, tn, tf);
         }
     }
+    const uint32_t inner = (1u << ni) - 1u;
+    uint32_t leafmask = hitbits & ((1u << nc) - 1u) & ~inner;
     // Triangle leaves: when some lane has two or more triangles to test, the
     // wave's pending tests are listed in LDS and dealt to its active lanes,
     // each testing another lane's ray (fetched by cross-lane permute) and
@@ -375,18 +398,9 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
         }
     }
     // nearest internal child next; the other hit internal children become
-    // one stack entry bounded by the nearest of them
-    uint32_t k0 = 0xFFFFFFFFu;
-#pragma unroll
-    for (int i = 0; i < 8; i++) k0 = min(k0, keys[i]);
+    // one stack entry bounded by the nearest of them (k1)
     if (k0 != 0xFFFFFFFFu) {
-        uint32_t k1 = 0xFFFFFFFFu, rest = 0;
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const bool other = keys[i] != 0xFFFFFFFFu && keys[i] != k0;
-            k1 = min(k1, other ? keys[i] : 0xFFFFFFFFu);
-            rest |= other ? (1u << (keys[i] & 7u)) : 0u;
-        }
+        const uint32_t rest = hitbits & inner & ~(1u << (k0 & 7u));
         if (rest && T.sp < LDS_DEPTH + SPILL_DEPTH) stack_push(st, T.sp, make_uint2(child_base, (k1 & ~0xFFu) | rest));
         T.node = child_base + (k0 & 7u);
         return false;
@@ -416,10 +430,13 @@ SHD void traverse(const DScene& S, const DRay& r, v3 inv, float& cat_t, int& cat
                   TraceStats& ts) {
     TravState T;
     T.node = 0; T.sp = 0; T.ct = cat_t; T.ci = cat_idx; T.ch = cat_hit;
+    trav_begin(S, CAT, r, inv, T);
     // decided once per traversal: the scene descriptor may be a per-lane
     // reference into global memory, read again on every node visit otherwise
     const bool share = CAT == C_TRI && S.wave_one_scene && S.count[C_TRI] < (1 << LEAF_SLOT_BITS);
-    while (!trav_step<CAT, COUNT>(S, CAT, ANY, r, inv, T, st, ts, share)) {
+    for (;;) {
+        if (COUNT) ts.witers += (__lane_id() == (uint32_t)(__ffsll((unsigned long long)__ballot(1)) - 1)) ? 1u : 0u;
+        if (trav_step<CAT, COUNT>(S, CAT, ANY, r, inv, T, st, ts, share)) break;
     }
     cat_t = T.ct; cat_idx = T.ci; cat_hit = T.ch;
 }
@@ -906,6 +923,7 @@ SHD float strands_depth(const DScene& S, const DRay& r, TStack& st) {
     line.tmax = __builtin_huge_valf();
     TravState T;
     T.node = 0; T.sp = 0; T.ct = __builtin_huge_valf(); T.ci = 0; T.ch = false;
+    trav_begin(S, C_STRAND, line, v.inv, T);
     TraceStats ts{0, 0};
     while (!trav_step<C_STRAND, false, StrandTau>(S, C_STRAND, false, line, v.inv, T, st, ts, false, &v)) {
     }

@@ -10,7 +10,9 @@
 static float decode(uint16_t q, uint8_t e, float o) { return std::fmaf(wb_q_value(q), wb_exp_scale(e), o); }
 
 // Checks a wide BVH over `count` primitive boxes (bounds: 6 floats each,
-// min xyz then max xyz): every primitive not flagged in skip (nullable) is
+// min xyz then max xyz): every node's slots hold its internal children
+// first, then its leaves, then empty slots, as its slot counts say; every
+// primitive not flagged in skip (nullable) is
 // referenced exactly once, every decoded child box lies inside the decoded
 // boxes of its ancestors, and every leaf's decoded box and all its
 // ancestors' contain the bounds of the leaf's primitives.  The quantiser
@@ -38,6 +40,14 @@ extern "C" int sheen_host_wbvh_check(const void* nodes_v, int n_nodes, const uns
         const WNode& n = nodes[it.node];
         const uint8_t ex[3] = {n.ex, n.ey, n.ez};
         const float org[3] = {n.ox, n.oy, n.oz};
+        // slot layout: internal children (meta = slot) first, then leaves, then empty slots
+        const int nc = n.slots & 15, ni = n.slots >> 4;
+        if (nc > 8 || ni > nc) violations++;
+        for (int i = 0; i < 8; i++) {
+            const uint8_t mi = n.meta[i];
+            const bool ok = i < ni ? mi == i : i < nc ? (mi != WBVH_EMPTY && (mi >> 5) != 0) : mi == WBVH_EMPTY;
+            if (!ok) violations++;
+        }
         for (int i = 0; i < 8; i++) {
             uint8_t m = n.meta[i];
             if (m == WBVH_EMPTY) continue;

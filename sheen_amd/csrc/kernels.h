@@ -96,6 +96,7 @@ struct Counters {
     unsigned long long nodes, prims, snodes, sprims, pnodes, pprims;
     unsigned long long reflection, refraction, extra_shadow;
     unsigned long long spills, sspills;
+    unsigned long long switers;  // node-visit iterations of the shadow kernel's waves
 };
 
 SHD uint32_t wave_sum(uint32_t v) {
@@ -524,9 +525,16 @@ __global__ SHADE_BOUNDS(FULL) void k_shade_primary(SRC src, SampleBufs SB, Event
         if (FULL && valid && SB.beam_add)
             beams_of(S, r, hit, hit ? rec.t : 0.0f, seed, SB.beam_add + (size_t)i * SB.n_beam, SB.n_beam, st, passes);
         if (FULL && valid && SB.media_p) SB.media_p[i] = media_of(S, r, hit ? rec.t : 1.0e9f, seed, st, passes);
-        const uint32_t a = block_reserve(&C->sec_n[0], want_r, want_t);
-        if (want_r) sec_q[a] = rr;
-        if (want_t) sec_q[a + (want_r ? 1u : 0u)] = tr;
+        if (EB.order & ORDER_SEC) {
+            uint32_t qa, qb;
+            block_reserve_ab(&C->sec_n[0], want_r, want_t, qa, qb);
+            if (want_r) sec_q[qa] = rr;
+            if (want_t) sec_q[qb] = tr;
+        } else {
+            const uint32_t a = block_reserve(&C->sec_n[0], want_r, want_t);
+            if (want_r) sec_q[a] = rr;
+            if (want_t) sec_q[a + (want_r ? 1u : 0u)] = tr;
+        }
         if (i < n) {
             sp.w = __int_as_float(flags);
             SB.samp[i] = sp;
@@ -698,6 +706,7 @@ __global__ TRACE_BOUNDS void k_shadow(SRC src, EventBufs EB, Counters* C, int pa
         wave_add64(&C->snodes, ts.nodes);
         wave_add64(&C->sprims, ts.prims);
         wave_add64(&C->sspills, st.spills);
+        wave_add64(&C->switers, ts.witers);
     }
 }
 

@@ -114,6 +114,8 @@ struct SheenContext {
     // one run shared by the lights; scenes shaded by the FULL kernels always
     // use one run
     bool light_runs = true;
+    // queue order of the shading kernels (EventBufs::order bits)
+    int order = ORDER_SHADOW | ORDER_SEC;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     struct Mark { int slot; hipEvent_t a, b; };
@@ -274,6 +276,44 @@ __global__ void k_tri_leaf(const uint32_t* prims, int n, const Triangle* tris, f
     out[3 * (size_t)k + 2] = make_float4(t.v2.x, t.v2.y, t.v2.z, 0.0f);
 }
 
+// Per axis, the largest 2|origin| + 2 WBVH_QMAX scale over `n` nodes, as
+// float bits (non-negative floats order as their bits; a NaN orders last
+// and sends every ray to the exact box test).
+__global__ void k_box_k(const WNode* nodes, int n, uint32_t* out) {
+    uint32_t m[3] = {0u, 0u, 0u};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const WNode& w = nodes[i];
+        const float o[3] = {w.ox, w.oy, w.oz};
+        const uint8_t e[3] = {w.ex, w.ey, w.ez};
+        for (int a = 0; a < 3; a++) {
+            const float v = 2.0f * fabsf(o[a]) + 2.0f * (float)WBVH_QMAX * __uint_as_float((uint32_t)e[a] << 23);
+            m[a] = max(m[a], __float_as_uint(v));
+        }
+    }
+    for (int a = 0; a < 3; a++) atomicMax(&out[a], m[a]);
+}
+
+// DScene::box_k of every category's BVH (the margins of the
+// distance-space box test, trav_begin).
+static bool scene_box_k(SheenContext* c) {
+    DScene& d = c->ds;
+    uint32_t* dk = nullptr;
+    uint32_t hk[N_BVH_CATS][3] = {};
+    if (!alloc_dev((void**)&dk, sizeof(hk))) { sheen_set_error("BVH bound alloc failed"); return false; }
+    hipError_t e = hipMemcpyAsync(dk, hk, sizeof(hk), hipMemcpyHostToDevice, c->stream);
+    for (int k = 0; k < N_BVH_CATS && e == hipSuccess; k++) {
+        if (!d.bvh[k] || c->bvh_nodes[k] <= 0) continue;
+        k_box_k<<<std::min(1024, (c->bvh_nodes[k] + 255) / 256), 256, 0, c->stream>>>(d.bvh[k], c->bvh_nodes[k], dk + 3 * k);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(hk, dk, sizeof(hk), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dk);
+    if (e != hipSuccess) { sheen_set_error("BVH bound failed: %s", hipGetErrorString(e)); return false; }
+    std::memcpy(d.box_k, hk, sizeof(hk));
+    return true;
+}
+
 // Builds every category BVH on the device from the primitive bounds (the
 // host's for all kinds but triangles, whose bounds come from the uploaded
 // records).  Same trees, byte for byte, as the host build.
@@ -381,6 +421,7 @@ static SheenContext* context_create(const SheenScene* in, int flags = 0) {
     } else if (!device_bvhs(c.get(), in, hs)) {
         return nullptr;
     }
+    if (!scene_box_k(c.get())) return nullptr;
     c->info.bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     d.objects = c->upload(in->objects, in->objects ? (size_t)in->num_objects : 0);
     d.strands = c->upload(in->sweep_strands, in->sweep_strands ? (size_t)in->num_sweep_strands : 0);
@@ -700,6 +741,7 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     EB.shadow_q = w.shadow_q; EB.shadow_nl = w.shadow_nl; EB.seg = 3u * (uint32_t)w.cap_slots;
     EB.L = c->ds.num_lights;
     EB.runs = (c->light_runs && !c->full) ? std::max(EB.L, 1) : 1;  // k_shadow<*, true> reads one run
+    EB.order = c->order;
     EB.cap = (uint32_t)w.cap_slots;
     SampleBufs SB;
     SB.hit_p = w.hit_p; SB.samp = w.samp; SB.beam_add = c->n_beam ? w.beam_add : nullptr;
@@ -761,6 +803,7 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     S.shadow_stack_spills = host.sspills;
     S.shadow_node_fetches = host.snodes;
     S.shadow_prim_tests = host.sprims;
+    S.shadow_visit_lanes = host.switers * 64ull;
     c->last = S;
     if (st) *st = S;
     return 0;
@@ -997,6 +1040,8 @@ int sheen_context_set_param(SheenContext* ctx, const char* name, int value) {
     if (value < 0) { sheen_set_error("negative parameter value"); return -1; }
     if (n == "overlap") ctx->overlap = value != 0;
     else if (n == "light_runs") ctx->light_runs = value != 0;
+    else if (n == "shadow_order") ctx->order = value ? (ctx->order | ORDER_SHADOW) : (ctx->order & ~ORDER_SHADOW);
+    else if (n == "sec_order") ctx->order = value ? (ctx->order | ORDER_SEC) : (ctx->order & ~ORDER_SEC);
     else if (n == "primary_grid") ctx->primary_grid = value;
     else if (n == "sec_grid") ctx->sec_grid = value;
     else if (n == "shadow_grid") ctx->shadow_grid = value;
@@ -1247,6 +1292,7 @@ struct SheenBatch {
     // caustics, media and the largest beam count size the shared buffers
     bool full = false, caustic = false, media = false;
     int n_beam = 0;
+    int order = ORDER_SHADOW | ORDER_SEC;  // as SheenContext::order
     size_t chunk_slots = 0;
     bool use_graph = true, dirty = true;
     int scene_table = SCENE_LDS;  // LDS entries of the slot -> scene search
@@ -1312,6 +1358,7 @@ static void batch_enqueue_t(SheenBatch* b) {
     EB.base = w.ev_base; EB.caustic = b->caustic ? w.ev_caustic : nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
     EB.shadow_q = w.shadow_q; EB.shadow_nl = w.shadow_nl; EB.seg = 3u * (uint32_t)w.cap_slots; EB.L = b->L;
     EB.runs = FULL ? 1 : std::max(b->L, 1);  // k_shadow<*, true> reads one run
+    EB.order = b->order;
     EB.cap = (uint32_t)w.cap_slots;
     SampleBufs SB;
     SB.hit_p = w.hit_p; SB.samp = w.samp;
@@ -1431,6 +1478,8 @@ int sheen_batch_set_param(SheenBatch* b, const char* name, int value) {
     else if (n == "shade_grid") b->shade_grid = value;
     else if (n == "shade_sec_grid") b->shade_sec_grid = value;
     else if (n == "graph") b->use_graph = value != 0;
+    else if (n == "shadow_order") b->order = value ? (b->order | ORDER_SHADOW) : (b->order & ~ORDER_SHADOW);
+    else if (n == "sec_order") b->order = value ? (b->order | ORDER_SEC) : (b->order & ~ORDER_SEC);
     else if (n == "scene_table") {
         if (value < 1 || value > SCENE_LDS) { sheen_set_error("scene_table must be 1..%d", SCENE_LDS); return -1; }
         b->scene_table = value;

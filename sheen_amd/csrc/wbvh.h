@@ -5,14 +5,16 @@
 //   [0,12)   quantisation origin (float x,y,z) = lower corner of the node box
 //   [12,15)  per-axis scale exponent bytes: scale = 2^(e-127), i.e. the float
 //            whose bit pattern is e<<23
-//   [15]     number of used child slots
+//   [15]     slot counts: low nibble = used child slots nc, high nibble =
+//            internal children ni; slots 0..ni-1 hold the internal children,
+//            ni..nc-1 the leaves, nc..7 are empty
 //   [16,20)  child_base: node index of the first internal child; internal
-//            children of a node are stored contiguously
+//            children of a node are stored contiguously, child_base + slot
 //   [20,24)  prim_base: first entry in the primitive reference array of the
 //            leaf children of this node (stored contiguously, slot order)
 //   [24,32)  meta[8]: 0xFF empty slot; (cnt<<5)|off with cnt in 1..7 for a
-//            leaf of cnt primitives at prim_base+off; (0<<5)|k for an
-//            internal child at child_base+k
+//            leaf of cnt primitives at prim_base+off; (0<<5)|k for the
+//            internal child in slot k, at child_base+k
 //   [32,128) quantised child boxes, SoA: lo_x[8] hi_x[8] lo_y[8] hi_y[8]
 //            lo_z[8] hi_z[8]; each an integer q in 0..WBVH_QMAX held as the
 //            bit pattern of the fp16 number q (exact)
@@ -29,7 +31,7 @@
 
 struct __attribute__((aligned(128))) WNode {
     float ox, oy, oz;
-    uint8_t ex, ey, ez, nchild;
+    uint8_t ex, ey, ez, slots;
     uint32_t child_base;
     uint32_t prim_base;
     uint8_t meta[8];

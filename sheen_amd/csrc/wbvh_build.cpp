@@ -142,13 +142,15 @@ void wbvh_build(const float* bmin, const float* bmax, int n, const unsigned char
             const BinNode& b = B.nodes[i];
             box = b.box; l = b.left; r = b.right; cnt = b.count;
         }, chv);
-        std::vector<int> ch(chv, chv + nch);
+        // internal children take the first slots, leaves the next ones
+        std::vector<int> ch;
+        for (int k = 0; k < nch; k++) if (B.nodes[chv[k]].count == 0) ch.push_back(chv[k]);
+        const int ni = (int)ch.size();
+        for (int k = 0; k < nch; k++) if (B.nodes[chv[k]].count > 0) ch.push_back(chv[k]);
         WNode wn;
         std::memset(&wn, 0, sizeof(wn));
         std::memset(wn.meta, 0xFF, sizeof(wn.meta));
-        wn.nchild = (uint8_t)ch.size();
-        int ni = 0;
-        for (int c : ch) if (B.nodes[c].count == 0) ni++;
+        wn.slots = (uint8_t)(ch.size() | ((size_t)ni << 4));
         wn.child_base = (uint32_t)out.nodes.size();
         wn.prim_base = (uint32_t)out.prims.size();
         WbBox cb[WBVH_WIDTH];

@@ -503,15 +503,19 @@ __global__ void k_wplan(const int* wl, int nw, const BNode* nodes, int* wch, uin
     if (w >= nw) return;
     int ch[WBVH_WIDTH];
     const int nch = wb_collapse(wl[w], [&](int i, WbBox& box, int& l, int& r, int& cnt) { node_view(nodes, i, box, l, r, cnt); }, ch);
+    // internal children take the first slots, leaves the next ones
     uint32_t a = 0, b = 0;
-    for (int k = 0; k < WBVH_WIDTH; k++) {
-        if (k < nch) {
-            const int c = nodes[ch[k]].count;
-            if (c > 0) b += (uint32_t)c;
-            else a++;
+    for (int k = 0; k < nch; k++)
+        if (nodes[ch[k]].count == 0) wch[w * WBVH_WIDTH + a++] = ch[k];
+    int at = (int)a;
+    for (int k = 0; k < nch; k++) {
+        const int c = nodes[ch[k]].count;
+        if (c > 0) {
+            b += (uint32_t)c;
+            wch[w * WBVH_WIDTH + at++] = ch[k];
         }
-        wch[w * WBVH_WIDTH + k] = k < nch ? ch[k] : -1;
     }
+    for (int k = nch; k < WBVH_WIDTH; k++) wch[w * WBVH_WIDTH + k] = -1;
     ni[w] = a;
     np[w] = b;
 }
@@ -545,7 +549,7 @@ __global__ void k_wemit(const int* wl, int nw, const BNode* nodes, const int* wc
             k_int++;
         }
     }
-    wn.nchild = (uint8_t)nch;
+    wn.slots = (uint8_t)(nch | (k_int << 4));
     if (!wb_quantize(wn, cb, nch)) atomicOr(&counters[3], 2);
     out[level_base + w] = wn;
     n_int[level_base + w] = (uint8_t)k_int;

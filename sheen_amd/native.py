@@ -21,7 +21,8 @@ class SheenRayStats(C.Structure):
                                           "shadow_node_fetches", "shadow_prim_tests", "primary_node_fetches",
                                           "primary_prim_tests", "primary_stack_spills", "shadow_stack_spills")] + \
                [("frame_ms", C.c_float), ("trace_ms", C.c_float), ("primary_ms", C.c_float), ("shadow_ms", C.c_float), ("shade_ms", C.c_float),
-                ("trace_launches", C.c_int), ("shadow_launches", C.c_int), ("shade_primary_ms", C.c_float)]
+                ("trace_launches", C.c_int), ("shadow_launches", C.c_int), ("shade_primary_ms", C.c_float),
+                ("shadow_visit_lanes", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -187,8 +187,9 @@ def _violations(nodes, prims, bounds, skip):
 
 def test_wide_bvh_check_catches_planted_boxes(scenes_mod):
     """A child box that reaches past its parent's decoded box on either side,
-    a leaf that loses a primitive, and a primitive referenced twice are each
-    counted as violations (the check itself is pinned, not only the trees)."""
+    a leaf that loses a primitive, a primitive referenced twice, and slots out
+    of the internal-first order are each counted as violations (the check
+    itself is pinned, not only the trees)."""
     sc = scenes_mod.generate(2, detail=24)
     nodes, prims, bounds, skip = _host_tree(sc, 2)
     assert len(nodes) > 8
@@ -232,6 +233,19 @@ def test_wide_bvh_check_catches_planted_boxes(scenes_mod):
     badp = prims.copy()
     badp[1] = badp[0]
     assert _violations(nodes, badp, bounds, skip) > 0
+    # slot layout: a node's last internal child and first leaf swapped (the
+    # traversal takes slot k for internal child k), and a wrong internal count
+    n = next(i for i in range(len(nodes)) if 0 < (nodes[i, 15] >> 4) < (nodes[i, 15] & 15))
+    ni = int(nodes[n, 15] >> 4)
+    bad = nodes.copy()
+    mb = bad[:, 24:32]
+    qb = bad[:, 32:128].view(np.uint16).reshape(-1, 6, 8)
+    mb[n, [ni - 1, ni]] = mb[n, [ni, ni - 1]]
+    qb[n, :, [ni - 1, ni]] = qb[n, :, [ni, ni - 1]]
+    assert _violations(bad, prims, bounds, skip) > 0
+    bad = nodes.copy()
+    bad[n, 15] = (bad[n, 15] & 15) | ((ni - 1) << 4)
+    assert _violations(bad, prims, bounds, skip) > 0
 
 
 def test_one_hip_runtime_when_torch_is_imported_first():
