@@ -199,6 +199,33 @@ def batch_leg(steps):
     return out
 
 
+def batch_cap_leg(reps=3):
+    """Where the batch speedup stops: batched against one-render-per-scene
+    wall time for the whole 256-scene batch, for its 1024x1024 scenes alone
+    and for the other scenes alone (medians of `reps` frames each).  The
+    large scenes fill the GPU on their own, so serial(all) / batched(large)
+    bounds the speedup even if the small scenes cost nothing."""
+    import statistics
+    from sheen_amd import scenes
+    from sheen_amd.batch import Batch
+    sc = scenes.batch(256)
+    out = {}
+    for name, keep in (("all", lambda s: True), ("large", lambda s: s.width == 1024),
+                       ("small", lambda s: s.width != 1024)):
+        sub = [s for s in sc if keep(s)]
+        b = Batch(sub)
+        b.render()
+        b.render_serial()
+        bt = statistics.median(b.render()["trace_ms"] for _ in range(reps))
+        st = statistics.median(b.render_serial()["trace_ms"] for _ in range(reps))
+        out[name] = {"scenes": len(sub), "samples": b.samples, "batched_ms": round(bt, 2), "serial_ms": round(st, 2),
+                     "speedup": round(st / bt, 3), "batched_ns_per_sample": round(bt * 1e6 / b.samples, 4),
+                     "serial_ns_per_sample": round(st * 1e6 / b.samples, 4)}
+        b.close()
+    out["speedup_bound_if_small_scenes_were_free"] = round(out["all"]["serial_ms"] / out["large"]["batched_ms"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,10 +236,15 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-scene batch leg")
     ap.add_argument("--no-analytic", action="store_true", help="skip the analytic-primitive (10k + CSG + caustics) leg")
     ap.add_argument("--detail", type=int, default=0, help="geodesic frequency override (default 229)")
+    ap.add_argument("--batch-cap", action="store_true",
+                    help="only measure the batch speedup bound (whole batch, large scenes, small scenes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this render path runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
+    if args.batch_cap:
+        print(json.dumps({"batch_cap": batch_cap_leg()}), flush=True)
+        return 0
 
     if not args.no_adjoint:
         # torch ships its own HIP runtime; it must initialise before this
@@ -300,6 +332,10 @@ def main():
             # what the counters say limits the kernel: "achieved" counts the
             # algorithmic node and triangle bytes, most of which the caches serve
             "counters": counters,
+            "note": "achieved = algorithmic bytes (128 B per node fetch, 48 B per triangle test, counted by the "
+                    "counting frame) per launch / launch time; frac above 1 means the caches serve most of them: "
+                    "counters.hbm_frac is the HBM traffic measured by FETCH_SIZE/WRITE_SIZE and counters.limiter "
+                    "the unit that bounds the kernel",
         },
         "kernel_ms_per_step": {k: round(tot[k] / args.steps, 3) for k in ("primary_ms", "trace_ms", "shadow_ms",
                                                                         "shade_ms")},

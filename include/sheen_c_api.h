@@ -232,8 +232,13 @@ int sheen_context_set_chunk(SheenContext* ctx, int samples_per_chunk);
  * (default 1: each light's shadow requests are one run of the queue, traced
  * light after light; 0: one run shared by the lights, in shading order; a
  * scene with primitives other than spheres, triangles and planes always uses
- * one run; the image is the same either way).  Returns 0, or -1 for an unknown name or a
- * negative value. */
+ * one run; the image is the same either way); "shadow_order" (default 1: a
+ * shading block queues its shadow requests that leave the surface before
+ * those that enter it) and "sec_order" (default 1: its reflection rays
+ * before its refraction rays), both only reorder queues; "shade_block"
+ * (workgroup size of the shading kernels, a multiple of 64 up to 1024; 0 =
+ * the compiled bound, 1024 for scenes of spheres, triangles and planes, 512
+ * otherwise).  Returns 0, or -1 for an unknown name or an invalid value. */
 int sheen_context_set_param(SheenContext* ctx, const char* name, int value);
 
 /* ---- Batches of scenes -------------------------------------------------------

@@ -9,8 +9,12 @@ queue depth, rays per thread; `render_kernel.cu:1240-1262` launches one
   `shade_sec_grid`; fewer blocks = more rays per thread);
 * queue depth     -> `chunk`, the samples of one wavefront chunk (every
   device queue is sized from it);
-* workgroup size and BVH width are compile-time: 256 threads (the LDS stack
-  columns are laid out per 256-lane block) and 8 children (one 128-byte node).
+* workgroup size -> `shade_block` for the shading kernels (up to the bound
+  they are compiled for); the traversal kernels' 256 threads are compile-time
+  (the LDS stack columns are laid out per 256-lane block), as is the BVH
+  width, 8 children (one 128-byte node);
+* queue order     -> `shadow_order`, `sec_order`, `light_runs` (which rays
+  share a wave of the trace kernels).
 
 `tune()` runs coordinate descent: each knob is swept with the others held,
 scored by the GPU time of the kernel it shapes (hipEvents on the render
@@ -26,13 +30,16 @@ SPACE = {
     "sec_grid": (4, 6, 8, 12, 16, 24),
     "shade_grid": (1, 2, 3, 4, 6, 8, 16),
     "shade_sec_grid": (1, 2, 4, 8, 16),
+    "shade_block": (1024, 512, 256),
     "overlap": (0, 1),
     "light_runs": (1, 0),
+    "shadow_order": (1, 0),
+    "sec_order": (1, 0),
 }
 
 # the library's built-in launch shapes (tuned on the 1M-triangle 1080p frame)
 DEFAULTS = {"primary_grid": 64, "sec_grid": 6, "shadow_grid": 64, "shade_grid": 1, "shade_sec_grid": 1, "chunk": 0,
-            "overlap": 0, "light_runs": 1}
+            "shade_block": 0, "overlap": 0, "light_runs": 1, "shadow_order": 1, "sec_order": 1}
 
 
 def _score(name, st):
@@ -47,8 +54,9 @@ def _score(name, st):
         return st["shade_primary_ms"]
     if name == "shade_sec_grid":
         return st["shade_ms"] - st["shade_primary_ms"]
-    # "overlap" runs kernels concurrently and "light_runs" reorders the shadow
-    # queue for the next shading too: only the frame time tells
+    # "overlap" runs kernels concurrently, and the workgroup size and the
+    # queue orders change which rays share a wave of the trace kernels: only
+    # the frame time tells
     return st["frame_ms"]
 
 
@@ -118,6 +126,9 @@ BATCH_SPACE = {
     "sec_grid": (4, 8, 16, 32),
     "shade_grid": (1, 2, 4, 8),
     "shade_sec_grid": (1, 2, 4, 8),
+    "shade_block": (1024, 512, 256),
+    "shadow_order": (1, 0),
+    "sec_order": (1, 0),
 }
 
 

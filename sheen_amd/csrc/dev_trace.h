@@ -329,7 +329,8 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
 #pragma unroll
             for (int b = 0; b < 6; b++) {
                 const uint64_t mb = __ballot((n >> b) & 1u);
-                off += (uint32_t)__popcll(mb & lt) << b;
+                // lanes below this one with bit b set (v_mbcnt)
+                off += __builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u)) << b;
                 N += (uint32_t)__popcll(mb) << b;
             }
             if (__ballot(n >= 2) != 0ull && N <= LEAF_LIST) {

@@ -91,6 +91,7 @@ struct SheenContext {
     // launch shapes: blocks per CU of each grid-stride kernel (0 = one work
     // item per thread); queue-fed kernels are launched over the queue capacity
     int primary_grid = 64, sec_grid = 6, shadow_grid = 64, shade_grid = 1, shade_sec_grid = 1, adjoint_grid = 8;
+    int shade_block = 0;  // workgroup size of the shading kernels (0: the compiled bound)
     // differentiable shading: why the scene is not differentiable ("" = it is),
     // float texel chains, their gradient, and the scene's own materials/lights
     std::string adj_reason;
@@ -180,6 +181,10 @@ static int grid_n(int num_cu, size_t n, int per_cu, int block = TRACE_BLOCK) {
     return (int)blocks;
 }
 static int grid_for(SheenContext* c, size_t n, int per_cu, int block = TRACE_BLOCK) { return grid_n(c->num_cu, n, per_cu, block); }
+// Workgroup size of the shading kernels: the knob's value (a multiple of 64)
+// up to the bound they are compiled for, which is also the default.
+template <bool FULL>
+static int shade_block_of(int knob) { return knob > 0 ? std::min(knob, (int)SHADE_BLOCK_OF(FULL)) : (int)SHADE_BLOCK_OF(FULL); }
 
 static bool alloc_dev(void** p, size_t bytes) {
     *p = nullptr;
@@ -620,7 +625,8 @@ static void run_chunk(SheenContext* c, WorkBufs& w, Counters* ctr, float4* frame
         }
         {
             Timer t(c, 4);
-            k_shade_primary<FULL><<<grid_for(c, F.n_slots, c->shade_grid, SHADE_BLOCK_OF(FULL)), SHADE_BLOCK_OF(FULL), 0, s>>>(src, SB, EB, w.sec_q[0], ctr);
+            const int sb = shade_block_of<FULL>(c->shade_block);
+            k_shade_primary<FULL><<<grid_for(c, F.n_slots, c->shade_grid, sb), sb, 0, s>>>(src, SB, EB, w.sec_q[0], ctr);
         }
         if (adj) {
             // replay: shadow transmittances of the primary hits, then the adjoint
@@ -662,7 +668,8 @@ static void run_chunk(SheenContext* c, WorkBufs& w, Counters* ctr, float4* frame
             }
             {
                 Timer t(c, 3);
-                k_shade_sec<FULL><<<grid_for(c, cap, c->shade_sec_grid, SHADE_BLOCK_OF(FULL)), SHADE_BLOCK_OF(FULL), 0, s>>>(src, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
+                const int sb = shade_block_of<FULL>(c->shade_block);
+                k_shade_sec<FULL><<<grid_for(c, cap, c->shade_sec_grid, sb), sb, 0, s>>>(src, SB, EB, w.sec_q[in_q], in_ctr, w.sec_hits,
                                                                        w.sec_q[out_q], out_ctr, ctr);
             }
         }
@@ -1047,6 +1054,10 @@ int sheen_context_set_param(SheenContext* ctx, const char* name, int value) {
     else if (n == "shadow_grid") ctx->shadow_grid = value;
     else if (n == "shade_grid") ctx->shade_grid = value;
     else if (n == "shade_sec_grid") ctx->shade_sec_grid = value;
+    else if (n == "shade_block") {
+        if (value % 64 != 0 || value > 1024) { sheen_set_error("shade_block must be a multiple of 64 up to 1024"); return -1; }
+        ctx->shade_block = value;
+    }
     else if (n == "adjoint_grid") ctx->adjoint_grid = value;
     else if (n == "chunk") ctx->chunk_override = value > 0 ? value : 0;
     else { sheen_set_error("unknown parameter: %s", name); return -1; }
@@ -1293,6 +1304,7 @@ struct SheenBatch {
     bool full = false, caustic = false, media = false;
     int n_beam = 0;
     int order = ORDER_SHADOW | ORDER_SEC;  // as SheenContext::order
+    int shade_block = 0;                   // as SheenContext::shade_block
     size_t chunk_slots = 0;
     bool use_graph = true, dirty = true;
     int scene_table = SCENE_LDS;  // LDS entries of the slot -> scene search
@@ -1352,7 +1364,7 @@ static void batch_enqueue_t(SheenBatch* b) {
     hipStream_t s = b->stream;
     WorkBufs& w = b->wb;
     const int cu = b->num_cu;
-    constexpr int SB_ = SHADE_BLOCK_OF(FULL);
+    const int SB_ = shade_block_of<FULL>(b->shade_block);
     k_zero_counters<<<1, 64, 0, s>>>(b->ctr);
     EventBufs EB;
     EB.base = w.ev_base; EB.caustic = b->caustic ? w.ev_caustic : nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
@@ -1477,6 +1489,10 @@ int sheen_batch_set_param(SheenBatch* b, const char* name, int value) {
     else if (n == "shadow_grid") b->shadow_grid = value;
     else if (n == "shade_grid") b->shade_grid = value;
     else if (n == "shade_sec_grid") b->shade_sec_grid = value;
+    else if (n == "shade_block") {
+        if (value % 64 != 0 || value > 1024) { sheen_set_error("shade_block must be a multiple of 64 up to 1024"); return -1; }
+        b->shade_block = value;
+    }
     else if (n == "graph") b->use_graph = value != 0;
     else if (n == "shadow_order") b->order = value ? (b->order | ORDER_SHADOW) : (b->order & ~ORDER_SHADOW);
     else if (n == "sec_order") b->order = value ? (b->order | ORDER_SEC) : (b->order & ~ORDER_SEC);

@@ -87,14 +87,17 @@ def test_chunking_does_not_change_the_image():
 
 @pytest.mark.parametrize("cfg", [1, 101, 102, 105])
 def test_launch_shape_does_not_change_the_image(cfg):
-    """Queue order depends on the grid; every queue entry carries its sample
-    slot, so the image is bit-identical for any launch shape."""
+    """Queue order depends on the grid, the shading workgroup size and the
+    queue-order knobs; every queue entry carries its sample slot, so the image
+    is bit-identical for any launch shape."""
     from sheen_amd import scenes, sheen, autotune
     sc = scenes.generate(cfg)
     shapes = [{k: v for k, v in autotune.DEFAULTS.items()},
               {"primary_grid": 1, "sec_grid": 1, "shadow_grid": 1, "shade_grid": 1, "shade_sec_grid": 1},
               {"primary_grid": 0, "sec_grid": 0, "shadow_grid": 0, "shade_grid": 0, "shade_sec_grid": 0,
-               "chunk": 64 * 4 * 5}]
+               "chunk": 64 * 4 * 5},
+              {"shade_block": 256, "shadow_order": 0, "sec_order": 0, "chunk": 0},
+              {"shade_block": 64, "shadow_order": 1, "sec_order": 1, "light_runs": 0}]
     with sheen.Context(sc.scene) as ctx:
         imgs = []
         for p in shapes:
@@ -142,6 +145,8 @@ def test_autotuner_picks_from_its_space_and_keeps_the_image():
             ctx.set_param("no_such_knob", 1)
         with pytest.raises(RuntimeError, match="negative"):
             ctx.set_param("shadow_grid", -1)
+        with pytest.raises(RuntimeError, match="shade_block"):
+            ctx.set_param("shade_block", 100)
     assert res["params"]["shadow_grid"] in (8, 64) and res["params"]["shade_grid"] in (1, 4)
     assert len(res["trials"]) == 4 and res["frame_ms"] > 0
     assert np.array_equal(img, ref)
