@@ -328,6 +328,9 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "bytes_per_launch": int(bytes_per_launch),
+            # counting frame: share of a shadow-kernel wave's lanes that visit a
+            # node per traversal iteration (the rest have finished their ray)
+            "shadow_traversal_lane_use": round(cst["shadow_node_fetches"] / max(1, cst["shadow_visit_lanes"]), 4),
             "avg_launch_ms": round(avg_launch_ms, 4),
             # what the counters say limits the kernel: "achieved" counts the
             # algorithmic node and triangle bytes, most of which the caches serve
