@@ -1,6 +1,8 @@
 """The distance-space child-box test of the wide-BVH traversal must contain
 the exact-path interval of every child (so the node test stays a superset
-of the reference slab test on primitive bounds).
+of the reference slab test on primitive bounds), with the per-ray margin
+E_a = 2^-20 |inv_a| (K_a + |ray_a|) + 2^-100 of trav_begin, K_a bounding
+2|o_a| + 2 Q s_a over the BVH's nodes.
 
 Child planes are integers q in 0..2048 held as fp16 patterns; the
 traversal's mixed-precision FMA takes q exactly.  Float32 emulation: + - *
@@ -27,12 +29,19 @@ def exact_path(q_lo, q_hi, o, s, ro, inv):
     return np.minimum(a, b), np.maximum(a, b)
 
 
+def box_k(o, s):
+    """The per-BVH bound of k_box_k (render.hip), float32: 2|o| + 2 Q s.
+    The traversal takes the largest over the nodes; the test gives every
+    node the smallest value that bound can have, its own."""
+    return (F(2.0) * np.abs(o) + (F(2.0 * QMAX) * s).astype(F)).astype(F)
+
+
 def distance_space(q_lo, q_hi, o, s, ro, inv):
     A = ((o - ro).astype(F) * inv).astype(F)
     B = (s * inv).astype(F)
-    e = fma32(fma32(F(QMAX) * np.ones_like(s), s, np.abs(o)), np.abs(inv),
-              fma32(F(QMAX) * np.ones_like(B), np.abs(B), np.abs(A)))
-    E = fma32(e, F(2.0 ** -20) * np.ones_like(e), F(2.0 ** -100) * np.ones_like(e))
+    # per-ray margin of trav_begin (dev_trace.h)
+    xa = (np.abs(inv) * (box_k(o, s) + np.abs(ro)).astype(F)).astype(F)
+    E = fma32(xa, F(2.0 ** -20) * np.ones_like(xa), F(2.0 ** -100) * np.ones_like(xa))
     pos = inv >= 0
     qn = np.where(pos, q_lo, q_hi)
     qf = np.where(pos, q_hi, q_lo)
