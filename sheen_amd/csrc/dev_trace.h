@@ -121,10 +121,11 @@ SHD bool leaf_dist(const DScene& S, uint32_t slot, const DRay& r, v3 inv, float&
     if (CAT == C_TRI) {
         float4 a = S.tri_leaf[3 * slot], b = S.tri_leaf[3 * slot + 1], c = S.tri_leaf[3 * slot + 2];
         idx = __float_as_int(a.w);
+        // both tests must pass; the gate runs only for a hit (most tests miss)
+        if (!t_triangle(r, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t)) return false;
         float mn[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z))};
         float mx[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z))};
-        if (!ref_slab(r, inv, mn, mx)) return false;
-        return t_triangle(r, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t);
+        return ref_slab(r, inv, mn, mx);
     }
     idx = (int)S.prim_ids[CAT][slot];
     const float* bb = S.bounds[CAT] + 6 * (size_t)idx;
