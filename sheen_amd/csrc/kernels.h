@@ -89,14 +89,30 @@ struct SecRay {
     uint32_t meta;  // bits 0..27 slot, 28 refraction, 29 entering, 30..31 face
 };
 
+// An event count that many waves add to: CTR_LINES partial sums, one
+// 128-byte line apart, each wave adding to the line of its block.
+// Device-scope atomics on one address are serialised past the per-XCD L2s,
+// and waves that finish together (the empty tail of a grid-stride launch)
+// queued on a single counter: the three secondary trace launches of the
+// 1M-triangle frame spent 0.23 ms on it.  The host adds the partial sums.
+#define CTR_LINES 16
+struct CtrSum {
+    unsigned long long line[CTR_LINES][16];
+};
+inline unsigned long long ctr_total(const CtrSum& c) {
+    unsigned long long t = 0;
+    for (int i = 0; i < CTR_LINES; i++) t += c.line[i][0];
+    return t;
+}
+
 struct Counters {
     uint32_t sec_n[4];       // secondary queues (ping-pong) sizes
-    uint32_t pad[4];         // (the shadow queue counts are per light: EventBufs::shadow_nl)
-    unsigned long long shadow_passes;
-    unsigned long long nodes, prims, snodes, sprims, pnodes, pprims;
-    unsigned long long reflection, refraction, extra_shadow;
-    unsigned long long spills, sspills;
-    unsigned long long switers;  // node-visit iterations of the shadow kernel's waves
+    uint32_t pad[28];        // (the shadow queue counts are per light: EventBufs::shadow_nl)
+    CtrSum shadow_passes;
+    CtrSum nodes, prims, snodes, sprims, pnodes, pprims;
+    CtrSum reflection, refraction, extra_shadow;
+    CtrSum spills, sspills;
+    CtrSum switers;  // node-visit iterations of the shadow kernel's waves
 };
 
 SHD uint32_t wave_sum(uint32_t v) {
@@ -106,6 +122,12 @@ SHD uint32_t wave_sum(uint32_t v) {
 SHD void wave_add64(unsigned long long* dst, uint32_t v) {
     uint32_t t = wave_sum(v);
     if (__lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1) atomicAdd(dst, (unsigned long long)t);
+}
+// A wave's total into its block's line of `dst`; a zero total adds nothing.
+SHD void wave_add64(CtrSum& dst, uint32_t v) {
+    const uint32_t t = wave_sum(v);
+    if (t != 0u && __lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1)
+        atomicAdd(&dst.line[blockIdx.x % CTR_LINES][0], (unsigned long long)t);
 }
 
 // Shadow test of one request (reference render_kernel.cu:237-280).
@@ -444,11 +466,11 @@ __global__ TRACE_BOUNDS void k_primary(SRC src, int4* hit_p, Counters* C) {
         hit_p[i] = out;
     }
     if (COUNT) {
-        wave_add64(&C->nodes, ts.nodes);
-        wave_add64(&C->prims, ts.prims);
-        wave_add64(&C->pnodes, ts.nodes);
-        wave_add64(&C->pprims, ts.prims);
-        wave_add64(&C->spills, st.spills);
+        wave_add64(C->nodes, ts.nodes);
+        wave_add64(C->prims, ts.prims);
+        wave_add64(C->pnodes, ts.nodes);
+        wave_add64(C->pprims, ts.prims);
+        wave_add64(C->spills, st.spills);
     }
 }
 
@@ -540,7 +562,7 @@ __global__ SHADE_BOUNDS(FULL) void k_shade_primary(SRC src, SampleBufs SB, Event
             SB.samp[i] = sp;
         }
     }
-    wave_add64(&C->extra_shadow, passes);
+    wave_add64(C->extra_shadow, passes);
 }
 
 template <bool COUNT, bool FULL, class SRC>
@@ -564,11 +586,11 @@ __global__ TRACE_BOUNDS void k_trace_sec(SRC src, const SecRay* q_in, int which,
         hits[j] = pack_hit(q);
         if (e.meta & (1u << 28)) nrefr++; else nrefl++;
     }
-    wave_add64(&C->reflection, nrefl);
-    wave_add64(&C->refraction, nrefr);
+    wave_add64(C->reflection, nrefl);
+    wave_add64(C->refraction, nrefr);
     if (COUNT) {
-        wave_add64(&C->nodes, ts.nodes);
-        wave_add64(&C->prims, ts.prims);
+        wave_add64(C->nodes, ts.nodes);
+        wave_add64(C->prims, ts.prims);
     }
 }
 
@@ -659,7 +681,7 @@ __global__ SHADE_BOUNDS(FULL) void k_shade_sec(SRC src, SampleBufs SB, EventBufs
         const uint32_t a = block_reserve(&C->sec_n[which_out], want_next, false);
         if (want_next) q_out[a] = nx;
     }
-    wave_add64(&C->extra_shadow, passes);
+    wave_add64(C->extra_shadow, passes);
 }
 
 template <bool COUNT, bool FULL, class SRC>
@@ -701,12 +723,12 @@ __global__ TRACE_BOUNDS void k_shadow(SRC src, EventBufs EB, Counters* C, int pa
         float tr = shadow_eval<COUNT, FULL>(S, q, st, passes, ts);
         EB.trans[q.slot & 0x7FFFFFFFu] = tr;
     }
-    wave_add64(&C->shadow_passes, passes);
+    wave_add64(C->shadow_passes, passes);
     if (COUNT) {
-        wave_add64(&C->snodes, ts.nodes);
-        wave_add64(&C->sprims, ts.prims);
-        wave_add64(&C->sspills, st.spills);
-        wave_add64(&C->switers, ts.witers);
+        wave_add64(C->snodes, ts.nodes);
+        wave_add64(C->sprims, ts.prims);
+        wave_add64(C->sspills, st.spills);
+        wave_add64(C->switers, ts.witers);
     }
 }
 
@@ -771,8 +793,9 @@ __global__ void k_reset(Counters* C, uint32_t* shadow_nl, int runs) {
 // Every counter of a frame to zero (a kernel, so that captured graphs hold no
 // memset nodes).
 __global__ void k_zero_counters(Counters* C) {
-    unsigned int* w = reinterpret_cast<unsigned int*>(C);
-    for (unsigned int i = threadIdx.x; i < sizeof(Counters) / 4u; i += blockDim.x) w[i] = 0u;
+    uint4* w = reinterpret_cast<uint4*>(C);
+    static_assert(sizeof(Counters) % 16 == 0, "counters clear in 16-byte words");
+    for (unsigned int i = threadIdx.x; i < sizeof(Counters) / 16u; i += blockDim.x) w[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 __global__ void k_clear_sec(Counters* C, int which) {
     if (threadIdx.x == 0 && blockIdx.x == 0) C->sec_n[which] = 0;

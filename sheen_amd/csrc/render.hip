@@ -755,12 +755,7 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     SB.media_p = c->ds.num_media ? w.media_p : nullptr; SB.media_r = c->ds.num_media ? w.media_r : nullptr;
     SB.n_beam = c->n_beam;
     hipStream_t s = c->stream;
-    if (fx) {
-        k_zero_counters<<<1, 64, 0, s>>>(ctr);
-    } else {
-        Counters zero{};
-        HIP_OR(hipMemcpyAsync(ctr, &zero, sizeof(Counters), hipMemcpyHostToDevice, s), -1);
-    }
+    k_zero_counters<<<1, 256, 0, s>>>(ctr);
     c->ev_used = 0;
     c->marks.clear();
     HIP_OR(hipEventRecord(c->ev_frame[0], s), -1);
@@ -797,20 +792,20 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     }
     S.samples = npix * ns;
     S.primary_rays = S.samples;
-    S.shadow_rays = host.shadow_passes + host.extra_shadow;
-    S.reflection_rays = host.reflection;
-    S.refraction_rays = host.refraction;
+    S.shadow_rays = ctr_total(host.shadow_passes) + ctr_total(host.extra_shadow);
+    S.reflection_rays = ctr_total(host.reflection);
+    S.refraction_rays = ctr_total(host.refraction);
     S.photon_rays = c->info.photon_rays;
     S.total_rays = S.primary_rays + S.shadow_rays + S.reflection_rays + S.refraction_rays;
-    S.node_fetches = host.nodes;
-    S.prim_tests = host.prims;
-    S.primary_node_fetches = host.pnodes;
-    S.primary_prim_tests = host.pprims;
-    S.primary_stack_spills = host.spills;
-    S.shadow_stack_spills = host.sspills;
-    S.shadow_node_fetches = host.snodes;
-    S.shadow_prim_tests = host.sprims;
-    S.shadow_visit_lanes = host.switers * 64ull;
+    S.node_fetches = ctr_total(host.nodes);
+    S.prim_tests = ctr_total(host.prims);
+    S.primary_node_fetches = ctr_total(host.pnodes);
+    S.primary_prim_tests = ctr_total(host.pprims);
+    S.primary_stack_spills = ctr_total(host.spills);
+    S.shadow_stack_spills = ctr_total(host.sspills);
+    S.shadow_node_fetches = ctr_total(host.snodes);
+    S.shadow_prim_tests = ctr_total(host.sprims);
+    S.shadow_visit_lanes = ctr_total(host.switers) * 64ull;
     c->last = S;
     if (st) *st = S;
     return 0;
@@ -1365,7 +1360,7 @@ static void batch_enqueue_t(SheenBatch* b) {
     WorkBufs& w = b->wb;
     const int cu = b->num_cu;
     const int SB_ = shade_block_of<FULL>(b->shade_block);
-    k_zero_counters<<<1, 64, 0, s>>>(b->ctr);
+    k_zero_counters<<<1, 256, 0, s>>>(b->ctr);
     EventBufs EB;
     EB.base = w.ev_base; EB.caustic = b->caustic ? w.ev_caustic : nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
     EB.shadow_q = w.shadow_q; EB.shadow_nl = w.shadow_nl; EB.seg = 3u * (uint32_t)w.cap_slots; EB.L = b->L;
@@ -1407,9 +1402,9 @@ static void batch_enqueue(SheenBatch* b) {
 static void batch_stats(SheenRayStats* S, const Counters& h, uint64_t samples) {
     S->samples = samples;
     S->primary_rays = samples;
-    S->shadow_rays = h.shadow_passes + h.extra_shadow;
-    S->reflection_rays = h.reflection;
-    S->refraction_rays = h.refraction;
+    S->shadow_rays = ctr_total(h.shadow_passes) + ctr_total(h.extra_shadow);
+    S->reflection_rays = ctr_total(h.reflection);
+    S->refraction_rays = ctr_total(h.refraction);
     S->total_rays = S->primary_rays + S->shadow_rays + S->reflection_rays + S->refraction_rays;
 }
 
