@@ -357,6 +357,8 @@ struct EventBufs {
     float4* contrib;  // [L][3 cap]: shade_direct of each light
     float* trans;     // [L][3 cap]: transmittance, written by the shadow kernel
     ShadowReq* shadow_q;
+    ShadowReq* cont[2];   // requests continuing past glass into their second / third pass
+    uint32_t cont_cap;    // entries of each (0: every pass in the first shadow kernel)
     uint32_t* shadow_nl;  // [2 runs]: requests queued per run, then their count after the primary shading
     uint32_t seg;         // queue entries per run (3 cap): run r's requests at r * seg
     int runs;             // queue runs: L (light l's requests are run l) or 1 (one queue shared by the lights)

@@ -107,7 +107,8 @@ inline unsigned long long ctr_total(const CtrSum& c) {
 
 struct Counters {
     uint32_t sec_n[4];       // secondary queues (ping-pong) sizes
-    uint32_t pad[28];        // (the shadow queue counts are per light: EventBufs::shadow_nl)
+    uint32_t cont_n[2];      // shadow continuations queued for the second and third pass
+    uint32_t pad[26];        // (the shadow queue counts are per light: EventBufs::shadow_nl)
     CtrSum shadow_passes;
     CtrSum nodes, prims, snodes, sprims, pnodes, pprims;
     CtrSum reflection, refraction, extra_shadow;
@@ -130,7 +131,40 @@ SHD void wave_add64(CtrSum& dst, uint32_t v) {
         atomicAdd(&dst.line[blockIdx.x % CTR_LINES][0], (unsigned long long)t);
 }
 
-// Shadow test of one request (reference render_kernel.cu:237-280).
+// One pass of the shadow test (reference render_kernel.cu:243-268): the
+// opaque query from r; 1 when nothing is hit, 0 when the hit blocks, 2 when
+// it is glass the ray passes (r.o then moved past it, t_max kept).
+template <bool COUNT, bool FULL>
+SHD int shadow_pass(const DScene& S, DRay& r, TStack& st, uint32_t& passes, TraceStats& ts) {
+    passes++;
+    QHit h;
+    if (!query_opaque<COUNT, FULL>(S, r, h, st, ts)) return 1;
+    const Material& m = S.materials[hit_material<FULL>(S, h, r)];
+    if (m.transmission > 0.5f) {
+        DHit full;
+        rebuild_hit<FULL>(S, r, h, full);
+        if (transmission_at(S, full.uvx, full.uvy, m) > 0.5f) {
+            r.o = full.point + r.d * 0.002f;
+            return 2;
+        }
+    }
+    return 0;
+}
+
+// Shadow test of one request (reference render_kernel.cu:237-280) from its
+// pass `first` on: at most three passes in all; after three glass passes the
+// ray is unblocked.
+template <bool COUNT, bool FULL>
+SHD float shadow_rest(const DScene& S, DRay& r, const DRay& orig, uint32_t slot, int first, TStack& st,
+                      uint32_t& passes, TraceStats& ts) {
+    for (int pass = first; pass < 3; pass++) {
+        const int res = shadow_pass<COUNT, FULL>(S, r, st, passes, ts);
+        if (res == 0) return 0.0f;
+        if (res == 1) break;
+    }
+    if (FULL && (slot & 0x80000000u)) return exp_r(-strands_depth(S, orig, st));
+    return 1.0f;
+}
 template <bool COUNT, bool FULL>
 SHD float shadow_eval(const DScene& S, const ShadowReq& q, TStack& st, uint32_t& passes, TraceStats& ts) {
     DRay r;
@@ -139,26 +173,7 @@ SHD float shadow_eval(const DScene& S, const ShadowReq& q, TStack& st, uint32_t&
     r.tmin = 0.001f;
     r.tmax = q.tmax;
     const DRay orig = r;
-    bool blocked = false;
-    for (int pass = 0; pass < 3; pass++) {
-        passes++;
-        QHit h;
-        if (!query_opaque<COUNT, FULL>(S, r, h, st, ts)) { blocked = false; break; }
-        const Material& m = S.materials[hit_material<FULL>(S, h, r)];
-        bool through = false;
-        if (m.transmission > 0.5f) {
-            DHit full;
-            rebuild_hit<FULL>(S, r, h, full);
-            if (transmission_at(S, full.uvx, full.uvy, m) > 0.5f) {
-                r.o = full.point + r.d * 0.002f;
-                through = true;
-            }
-        }
-        if (!through) { blocked = true; break; }
-    }
-    if (blocked) return 0.0f;
-    if (FULL && (q.slot & 0x80000000u)) return exp_r(-strands_depth(S, orig, st));
-    return 1.0f;
+    return shadow_rest<COUNT, FULL>(S, r, orig, q.slot, 0, st, passes, ts);
 }
 
 // Occlusion query used by beams and media (scene_intersect_opaque only).
@@ -720,8 +735,74 @@ __global__ TRACE_BOUNDS void k_shadow(SRC src, EventBufs EB, Counters* C, int pa
         ShadowReq q = EB.shadow_q[at];
         // the request's index of its (event, light) entry gives the sample slot
         const DScene& S = src.trace_scene(src.find(ev_slot(EB, q.slot & 0x7FFFFFFFu)));
+        if (!FULL && EB.cont_cap) {
+            // first pass here; a ray that passes glass continues in
+            // k_shadow_cont, in waves of such rays (queue full: here)
+            DRay r;
+            r.o = mk3(q.ox, q.oy, q.oz);
+            r.d = mk3(q.dx, q.dy, q.dz);
+            r.tmin = 0.001f;
+            r.tmax = q.tmax;
+            const int res = shadow_pass<COUNT, false>(S, r, st, passes, ts);
+            const uint32_t k = wave_reserve(&C->cont_n[0], res == 2);
+            if (res != 2) {
+                EB.trans[q.slot & 0x7FFFFFFFu] = (res == 1) ? 1.0f : 0.0f;
+            } else if (k < EB.cont_cap) {
+                ShadowReq c = q;
+                c.ox = r.o.x; c.oy = r.o.y; c.oz = r.o.z;
+                EB.cont[0][k] = c;
+            } else {
+                EB.trans[q.slot & 0x7FFFFFFFu] = shadow_rest<COUNT, false>(S, r, r, q.slot, 1, st, passes, ts);
+            }
+            continue;
+        }
         float tr = shadow_eval<COUNT, FULL>(S, q, st, passes, ts);
         EB.trans[q.slot & 0x7FFFFFFFu] = tr;
+    }
+    wave_add64(C->shadow_passes, passes);
+    if (COUNT) {
+        wave_add64(C->snodes, ts.nodes);
+        wave_add64(C->sprims, ts.prims);
+        wave_add64(C->sspills, st.spills);
+        wave_add64(C->switers, ts.witers);
+    }
+}
+
+// Second (stage 0) and third (stage 1) shadow pass of the requests whose
+// earlier pass went through glass, queued by k_shadow / the stage before
+// (spheres, triangles and planes only).  Tracing them in waves of their own
+// keeps the first-pass waves from idling while a few of their lanes cross the
+// glass; each request still sees the same passes from the same origins.
+template <bool COUNT, class SRC>
+__global__ TRACE_BOUNDS void k_shadow_cont(SRC src, EventBufs EB, Counters* C, int stage) {
+    SCENE_TABLE(src);
+    SHEEN_STACK(st);
+    TraceStats ts{0, 0, 0};
+    const uint32_t n = min(C->cont_n[stage], EB.cont_cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t passes = 0;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const ShadowReq q = EB.cont[stage][j];
+        const uint32_t slot = q.slot & 0x7FFFFFFFu;
+        const DScene& S = src.trace_scene(src.find(ev_slot(EB, slot)));
+        DRay r;
+        r.o = mk3(q.ox, q.oy, q.oz);
+        r.d = mk3(q.dx, q.dy, q.dz);
+        r.tmin = 0.001f;
+        r.tmax = q.tmax;
+        const int res = shadow_pass<COUNT, false>(S, r, st, passes, ts);
+        const uint32_t k = wave_reserve(&C->cont_n[1], stage == 0 && res == 2);
+        if (res != 2) {
+            EB.trans[slot] = (res == 1) ? 1.0f : 0.0f;
+        } else if (stage == 1) {
+            EB.trans[slot] = 1.0f;  // three glass passes: unblocked
+        } else if (k < EB.cont_cap) {
+            ShadowReq c = q;
+            c.ox = r.o.x; c.oy = r.o.y; c.oz = r.o.z;
+            EB.cont[1][k] = c;
+        } else {
+            EB.trans[slot] = shadow_rest<COUNT, false>(S, r, r, q.slot, 2, st, passes, ts);
+        }
     }
     wave_add64(C->shadow_passes, passes);
     if (COUNT) {
@@ -786,8 +867,10 @@ __global__ void k_snap_shadow(uint32_t* shadow_nl, int runs) {
     for (int l = threadIdx.x; l < runs; l += blockDim.x) shadow_nl[runs + l] = shadow_nl[l];
 }
 __global__ void k_reset(Counters* C, uint32_t* shadow_nl, int runs) {
-    if (threadIdx.x == 0 && blockIdx.x == 0)
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
         for (int k = 0; k < 4; k++) C->sec_n[k] = 0;
+        C->cont_n[0] = C->cont_n[1] = 0;
+    }
     for (int l = threadIdx.x; l < runs; l += blockDim.x) shadow_nl[l] = 0;
 }
 // Every counter of a frame to zero (a kernel, so that captured graphs hold no

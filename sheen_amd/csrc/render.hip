@@ -57,6 +57,7 @@ struct WorkBufs {
     float4* contrib = nullptr;
     float* trans = nullptr;
     ShadowReq* shadow_q = nullptr;
+    ShadowReq* cont[2] = {nullptr, nullptr};  // shadow continuations (second, third pass)
     uint32_t* shadow_nl = nullptr;  // per-light request counts and their snapshot
     SecRay* sec_q[2] = {nullptr, nullptr};
     int4* sec_hits = nullptr;
@@ -64,8 +65,8 @@ struct WorkBufs {
     float4* media_p = nullptr;
     float4* media_r = nullptr;
     void release() {
-        void* ps[] = {hit_p, samp, ev_base, ev_caustic, contrib, trans, shadow_q, shadow_nl, sec_q[0], sec_q[1], sec_hits,
-                      beam_add, media_p, media_r};
+        void* ps[] = {hit_p, samp, ev_base, ev_caustic, contrib, trans, shadow_q, cont[0], cont[1], shadow_nl, sec_q[0],
+                      sec_q[1], sec_hits, beam_add, media_p, media_r};
         for (void* p : ps) if (p) (void)hipFree(p);
         *this = WorkBufs();
     }
@@ -530,10 +531,15 @@ static SheenContext* context_create(const SheenScene* in, int flags = 0) {
 }
 
 // Per-slot bytes of the work buffers.
+// Entries of each shadow continuation queue for `shadow_entries` requests: a
+// quarter of them (requests beyond it finish their passes in the first
+// shadow kernel).
+static size_t cont_entries(size_t shadow_entries) { return std::max<size_t>(64, shadow_entries / 4); }
+
 static size_t slot_bytes(int L, int n_beam, bool caustic, bool media) {
     size_t b = 16 /*hit*/ + 16 /*samp*/ + 3 * 16 /*base*/ + (caustic ? 3 * 16 : 0) + 3 * (size_t)L * (16 + 4) +
                2 * 32 /*sec queues*/ + 2 * 16 /*sec hits*/ + 3 * (size_t)L * 32 /*shadow queue*/ +
-               (size_t)n_beam * 16 + (media ? 32 : 0);
+               2 * 3 * (size_t)L * 32 / 4 /*shadow continuations*/ + (size_t)n_beam * 16 + (media ? 32 : 0);
     return b;
 }
 
@@ -555,6 +561,8 @@ static bool ensure_bufs(WorkBufs& w, size_t slots, int L, int n_beam, bool caust
               alloc_dev((void**)&w.contrib, ev * Lq * sizeof(float4)) &&
               alloc_dev((void**)&w.trans, ev * Lq * sizeof(float)) &&
               alloc_dev((void**)&w.shadow_q, ev * Lq * sizeof(ShadowReq)) &&
+              alloc_dev((void**)&w.cont[0], cont_entries(ev * Lq) * sizeof(ShadowReq)) &&
+              alloc_dev((void**)&w.cont[1], cont_entries(ev * Lq) * sizeof(ShadowReq)) &&
               alloc_dev((void**)&w.shadow_nl, 2 * Lq * sizeof(uint32_t)) &&
               alloc_dev((void**)&w.sec_q[0], 2 * slots * sizeof(SecRay)) &&
               alloc_dev((void**)&w.sec_q[1], 2 * slots * sizeof(SecRay)) &&
@@ -604,6 +612,19 @@ struct AdjCall {
     AdjGrads G;
 };
 
+// The second and third shadow passes of the requests that went through
+// glass (scenes of spheres, triangles and planes; the others keep every pass
+// in k_shadow).
+template <bool FULL, class SRC>
+static void launch_shadow_cont(hipStream_t s, int grid, bool counting, const SRC& src, const EventBufs& EB,
+                               Counters* ctr) {
+    if (FULL || !EB.cont_cap) return;
+    for (int stage = 0; stage < 2; stage++) {
+        if (counting) k_shadow_cont<true><<<grid, TRACE_BLOCK, 0, s>>>(src, EB, ctr, stage);
+        else k_shadow_cont<false><<<grid, TRACE_BLOCK, 0, s>>>(src, EB, ctr, stage);
+    }
+}
+
 template <bool FULL>
 static void run_chunk(SheenContext* c, WorkBufs& w, Counters* ctr, float4* frame, const FrameDesc& F, SampleBufs& SB,
                       EventBufs& EB, SheenRayStats& S, bool hits_only, int* dk, int* di, float* dt, const AdjCall* adj) {
@@ -633,6 +654,7 @@ static void run_chunk(SheenContext* c, WorkBufs& w, Counters* ctr, float4* frame
             // (differentiable scenes have no secondary rays)
             const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
             k_shadow<false, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(src, EB, ctr, 0);
+            launch_shadow_cont<FULL>(s, grid_for(c, EB.cont_cap, c->shadow_grid), false, src, EB, ctr);
             S.shadow_launches++;
             k_adjoint<<<grid_for(c, F.n_slots, c->adjoint_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, EB, adj->grad, adj->G);
             return;
@@ -682,6 +704,11 @@ static void run_chunk(SheenContext* c, WorkBufs& w, Counters* ctr, float4* frame
             S.shadow_launches++;
         }
         if (overlap) (void)hipStreamWaitEvent(s, c->ev_side[1], 0);
+        {
+            // the continuations of both shadow launches (one queue)
+            Timer t(c, 2);
+            launch_shadow_cont<FULL>(s, grid_for(c, EB.cont_cap, c->shadow_grid), c->counting, src, EB, ctr);
+        }
         {
             Timer t(c, 3);
             k_accumulate<<<grid_for(c, F.n_slots / ns, 8), 256, 0, s>>>(src, SB, EB, frame);
@@ -746,6 +773,8 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     EventBufs EB;
     EB.base = w.ev_base; EB.caustic = caustic ? w.ev_caustic : nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
     EB.shadow_q = w.shadow_q; EB.shadow_nl = w.shadow_nl; EB.seg = 3u * (uint32_t)w.cap_slots;
+    EB.cont[0] = w.cont[0]; EB.cont[1] = w.cont[1];
+    EB.cont_cap = c->full ? 0u : (uint32_t)cont_entries(3 * w.cap_slots * (size_t)std::max(w.cap_L, 1));
     EB.L = c->ds.num_lights;
     EB.runs = (c->light_runs && !c->full) ? std::max(EB.L, 1) : 1;  // k_shadow<*, true> reads one run
     EB.order = c->order;
@@ -866,6 +895,8 @@ static size_t carve_work(unsigned char* base, size_t slots, int L, int n_beam, b
     w.contrib = (float4*)take(ev * Lq * sizeof(float4));
     w.trans = (float*)take(ev * Lq * sizeof(float));
     w.shadow_q = (ShadowReq*)take(ev * Lq * sizeof(ShadowReq));
+    w.cont[0] = (ShadowReq*)take(cont_entries(ev * Lq) * sizeof(ShadowReq));
+    w.cont[1] = (ShadowReq*)take(cont_entries(ev * Lq) * sizeof(ShadowReq));
     w.shadow_nl = (uint32_t*)take(2 * Lq * sizeof(uint32_t));
     w.sec_q[0] = (SecRay*)take(2 * slots * sizeof(SecRay));
     w.sec_q[1] = (SecRay*)take(2 * slots * sizeof(SecRay));
@@ -1364,6 +1395,8 @@ static void batch_enqueue_t(SheenBatch* b) {
     EventBufs EB;
     EB.base = w.ev_base; EB.caustic = b->caustic ? w.ev_caustic : nullptr; EB.contrib = w.contrib; EB.trans = w.trans;
     EB.shadow_q = w.shadow_q; EB.shadow_nl = w.shadow_nl; EB.seg = 3u * (uint32_t)w.cap_slots; EB.L = b->L;
+    EB.cont[0] = w.cont[0]; EB.cont[1] = w.cont[1];
+    EB.cont_cap = FULL ? 0u : (uint32_t)cont_entries(3 * w.cap_slots * (size_t)std::max(w.cap_L, 1));
     EB.runs = FULL ? 1 : std::max(b->L, 1);  // k_shadow<*, true> reads one run
     EB.order = b->order;
     EB.cap = (uint32_t)w.cap_slots;
@@ -1390,6 +1423,7 @@ static void batch_enqueue_t(SheenBatch* b) {
                 src, SB, EB, w.sec_q[in_q], round, w.sec_hits, w.sec_q[out_q], round + 1, b->ctr);
         }
         k_shadow<false, FULL><<<grid_n(cu, 3 * n * (size_t)b->L, b->shadow_grid), TRACE_BLOCK, 0, s>>>(src, EB, b->ctr, 0);
+        launch_shadow_cont<FULL>(s, grid_n(cu, EB.cont_cap, b->shadow_grid), false, src, EB, b->ctr);
         k_accumulate<<<grid_n(cu, n, 8), 256, 0, s>>>(src, SB, EB, b->frame);
     }
 }
