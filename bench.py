@@ -236,6 +236,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-scene batch leg")
     ap.add_argument("--no-analytic", action="store_true", help="skip the analytic-primitive (10k + CSG + caustics) leg")
     ap.add_argument("--detail", type=int, default=0, help="geodesic frequency override (default 229)")
+    ap.add_argument("--leg", choices=("analytic", "batch", "adjoint"),
+                    help="run only this secondary leg and print its JSON object (a workload to profile on its own)")
     ap.add_argument("--batch-cap", action="store_true",
                     help="only measure the batch speedup bound (whole batch, large scenes, small scenes)")
     args = ap.parse_args()
@@ -244,6 +246,15 @@ def main():
         return 2
     if args.batch_cap:
         print(json.dumps({"batch_cap": batch_cap_leg()}), flush=True)
+        return 0
+    if args.leg:
+        if args.leg == "adjoint":
+            import torch
+            torch.cuda.init()
+        leg = {"analytic": lambda: analytic_leg(max(2, min(args.steps, 5))),
+               "batch": lambda: batch_leg(max(2, min(args.steps, 3))),
+               "adjoint": lambda: adjoint_leg(max(3, min(args.steps, 10)))}[args.leg]
+        print(json.dumps({args.leg: leg()}), flush=True)
         return 0
 
     if not args.no_adjoint:

@@ -168,21 +168,83 @@ SHD void wave_add(WaveTable& t, float* dst, bool active, uint32_t e, v3 val, boo
     }
 }
 
+// Texel gradients of one 8x8 tile, summed in LDS before they go out: an
+// open-addressed table of ADJ_TEX_SLOTS element keys (the chain offset of a
+// texel's first channel) with three float sums each.  Every sample of a tile
+// taps a small, overlapping patch of texels (the bilinear 2x2 of each of the
+// tile's samples), so a texel that tens of samples touch takes one
+// device-scope atomic per channel instead of one per sample.
+#define ADJ_TEX_BITS 10
+#define ADJ_TEX_SLOTS (1 << ADJ_TEX_BITS)
+#define ADJ_TEX_PROBES 32
+struct TexSums {
+    uint32_t* key;  // 0xFFFFFFFF = free
+    float* x;
+    float* y;
+    float* z;
+};
+SHD void tex_sums_add(const TexSums& T, float* dst, bool active, uint32_t e, v3 v) {
+    if (!active || (v.x == 0.0f && v.y == 0.0f && v.z == 0.0f)) return;
+    uint32_t h = (e * 2654435761u) >> (32 - ADJ_TEX_BITS);
+    for (int p = 0; p < ADJ_TEX_PROBES; p++) {
+        const uint32_t old = atomicCAS(&T.key[h], 0xFFFFFFFFu, e);
+        if (old == 0xFFFFFFFFu || old == e) {
+            atomicAdd(&T.x[h], v.x);
+            atomicAdd(&T.y[h], v.y);
+            atomicAdd(&T.z[h], v.z);
+            return;
+        }
+        h = (h + 1u) & (ADJ_TEX_SLOTS - 1u);
+    }
+    // table crowded: straight to the gradient buffer
+    if (v.x != 0.0f) unsafeAtomicAdd(dst + e, v.x);
+    if (v.y != 0.0f) unsafeAtomicAdd(dst + e + 1, v.y);
+    if (v.z != 0.0f) unsafeAtomicAdd(dst + e + 2, v.z);
+}
+// Every entry out to dst and the table cleared; block-wide (barriers inside).
+SHD void tex_sums_flush(const TexSums& T, float* dst) {
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < ADJ_TEX_SLOTS; j += blockDim.x) {
+        const uint32_t k = T.key[j];
+        if (k == 0xFFFFFFFFu) continue;
+        if (T.x[j] != 0.0f) unsafeAtomicAdd(dst + k, T.x[j]);
+        if (T.y[j] != 0.0f) unsafeAtomicAdd(dst + k + 1, T.y[j]);
+        if (T.z[j] != 0.0f) unsafeAtomicAdd(dst + k + 2, T.z[j]);
+        T.key[j] = 0xFFFFFFFFu;
+        T.x[j] = T.y[j] = T.z[j] = 0.0f;
+    }
+    __syncthreads();
+}
+
 // Gradient of sum_pixels <grad_img, pixel> for one chunk of primary samples.
 // Requires the chunk's primary hits (k_primary) and transmittances
-// (k_shade_primary + k_shadow) of the replayed frame.
+// (k_shade_primary + k_shadow) of the replayed frame.  A block takes whole
+// 8x8 tiles (the tile's 64 * ns slots are contiguous), so that its texel
+// table sums every sample of the tile before it is flushed.
 __global__ ADJOINT_BOUNDS void k_adjoint(DScene S, FrameDesc F, const int4* hit_p, EventBufs EB, const float* grad_img,
                                        AdjGrads G) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t n_iter = (F.n_slots + stride - 1) / stride;  // whole waves reach every ballot
+    __shared__ uint32_t s_key[ADJ_TEX_SLOTS];
+    __shared__ float s_x[ADJ_TEX_SLOTS], s_y[ADJ_TEX_SLOTS], s_z[ADJ_TEX_SLOTS];
+    const TexSums T{s_key, s_x, s_y, s_z};
+    for (uint32_t j = threadIdx.x; j < ADJ_TEX_SLOTS; j += blockDim.x) {
+        s_key[j] = 0xFFFFFFFFu;
+        s_x[j] = s_y[j] = s_z[j] = 0.0f;
+    }
+    __syncthreads();
+    const uint32_t tile_slots = 64u * (uint32_t)F.ns;
+    const uint32_t n_tiles = F.n_slots / tile_slots;
+    // whole waves (the block) reach every ballot: a uniform count per tile
+    const uint32_t per_tile = (tile_slots + blockDim.x - 1) / blockDim.x;
     const int L = S.num_lights;
     const float inv_ns = 1.0f / (float)F.ns;
-    WaveTable t_em, t_alb, t_int, t_tex;
-    table_init(t_em); table_init(t_alb); table_init(t_int); table_init(t_tex);
-    for (uint32_t it = 0; it < n_iter; it++) {
-        const uint32_t i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    WaveTable t_em, t_alb, t_int;
+    table_init(t_em); table_init(t_alb); table_init(t_int);
+    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    for (uint32_t it = 0; it < per_tile; it++) {
+        const uint32_t o = it * blockDim.x + threadIdx.x;
+        const uint32_t i = tile * tile_slots + o;
         int x = 0, y = 0, s = 0;
-        const bool valid = i < F.n_slots && slot_pixel(F, F.g0 + i, x, y, s);
+        const bool valid = o < tile_slots && slot_pixel(F, F.g0 + i, x, y, s);
         float pa = 0.0f;
         uint32_t seed = 0;
         DRay r;
@@ -228,27 +290,23 @@ __global__ ADJOINT_BOUNDS void k_adjoint(DScene S, FrameDesc F, const int4* hit_
             table_add(t_int, G.intensity, (uint32_t)l, dI, 0.0f, 0.0f);
         }
         const bool textured = hit && tex_ok(S, mat.texture_id);
-        TexTap p;
-        p.e00 = p.e10 = p.e01 = p.e11 = 0;
-        p.tx = p.ty = 0.0f;
-        uint32_t base_e = 0;  // the texture's chain offset: keys are absolute
-        v3 gb = mk3(0, 0, 0);
         if (textured) {
             const TextureRef& t = S.textures[mat.texture_id];
-            p = tex_tap(t, in.uvx, in.uvy, tex_lod(t, in.uv_fp));
-            gb = g * dbase;
-            base_e = S.tex_f_off[mat.texture_id];
+            const TexTap p = tex_tap(t, in.uvx, in.uvy, tex_lod(t, in.uv_fp));
+            const v3 gb = g * dbase;
+            const uint32_t base_e = S.tex_f_off[mat.texture_id];  // the texture's chain offset: keys are absolute
+            tex_sums_add(T, G.tex, true, base_e + p.e00, gb * ((1.0f - p.tx) * (1.0f - p.ty)));
+            tex_sums_add(T, G.tex, true, base_e + p.e10, gb * (p.tx * (1.0f - p.ty)));
+            tex_sums_add(T, G.tex, true, base_e + p.e01, gb * ((1.0f - p.tx) * p.ty));
+            tex_sums_add(T, G.tex, true, base_e + p.e11, gb * (p.tx * p.ty));
         }
-        wave_add(t_tex, G.tex, textured, base_e + p.e00, gb * ((1.0f - p.tx) * (1.0f - p.ty)), false);
-        wave_add(t_tex, G.tex, textured, base_e + p.e10, gb * (p.tx * (1.0f - p.ty)), false);
-        wave_add(t_tex, G.tex, textured, base_e + p.e01, gb * ((1.0f - p.tx) * p.ty), false);
-        wave_add(t_tex, G.tex, textured, base_e + p.e11, gb * (p.tx * p.ty), false);
         const uint32_t mk = hit ? 3u * (uint32_t)rec.material_id : 0u;
         wave_add(t_em, G.emission, hit, mk, g, false);
         wave_add(t_alb, G.albedo, hit && !textured, mk, g * dbase, false);
     }
+        tex_sums_flush(T, G.tex);
+    }
     table_flush(t_em, G.emission);
     table_flush(t_alb, G.albedo);
     table_flush(t_int, G.intensity);
-    table_flush(t_tex, G.tex);
 }

@@ -656,7 +656,9 @@ static void run_chunk(SheenContext* c, WorkBufs& w, Counters* ctr, float4* frame
             k_shadow<false, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(src, EB, ctr, 0);
             launch_shadow_cont<FULL>(s, grid_for(c, EB.cont_cap, c->shadow_grid), false, src, EB, ctr);
             S.shadow_launches++;
-            k_adjoint<<<grid_for(c, F.n_slots, c->adjoint_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, EB, adj->grad, adj->G);
+            // one block per 8x8 tile of samples at a time
+            const size_t tiles = F.n_slots / (64u * (uint32_t)ns);
+            k_adjoint<<<grid_for(c, tiles * TRACE_BLOCK, c->adjoint_grid), TRACE_BLOCK, 0, s>>>(c->ds, F, w.hit_p, EB, adj->grad, adj->G);
             return;
         }
         // The primary shading's shadow requests are complete here: trace them
