@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s, 1M-tri 1080p; batched-vs-serial scene speedup; adjoint/forward time"
 HBM_PEAK_GBS = 8000.0
+PARAMS = {}  # --param knobs
 NODE_BYTES = 128   # one wide BVH node per fetch
 TRI_LEAF_BYTES = 48  # vertices + index of one triangle leaf slot
 # the device code the committed counter profiles were measured on
@@ -153,6 +154,8 @@ def analytic_leg(steps):
     # the primary shading's shadow rays on a second stream during the
     # secondary rays: faster on this workload (autotuner knob "overlap")
     ctx.set_param("overlap", 1)
+    for k, v in PARAMS.items():
+        ctx.set_param(k, v)
     W, H, A = sc.width, sc.height, sc.aa
     ctx.render(W, H, A, download=False)
     rays = 0
@@ -181,6 +184,8 @@ def batch_leg(steps):
     from sheen_amd.batch import Batch
     sc = scenes.batch(256)
     b = Batch(sc)
+    for k, v in PARAMS.items():
+        b.set_param(k, v)
     b.render()
     b.render_serial()
     bt = st = 0.0
@@ -236,11 +241,19 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-scene batch leg")
     ap.add_argument("--no-analytic", action="store_true", help="skip the analytic-primitive (10k + CSG + caustics) leg")
     ap.add_argument("--detail", type=int, default=0, help="geodesic frequency override (default 229)")
+    ap.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
+                    help="launch or queue-order knob (sheen_context_set_param / sheen_batch_set_param) for the "
+                         "mesh, analytic and batch workloads; repeatable")
     ap.add_argument("--leg", choices=("analytic", "batch", "adjoint"),
                     help="run only this secondary leg and print its JSON object (a workload to profile on its own)")
     ap.add_argument("--batch-cap", action="store_true",
                     help="only measure the batch speedup bound (whole batch, large scenes, small scenes)")
     args = ap.parse_args()
+    global PARAMS
+    PARAMS = {}
+    for kv in args.param:
+        k, _, v = kv.partition("=")
+        PARAMS[k] = int(v)
     if args.gpus != 1:
         print("bench.py: this render path runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
@@ -274,6 +287,8 @@ def main():
         host_build_ms = hctx.build_ms
     ctx = sheen.Context(sc.scene)
     setup = {"device_bvh_ms": round(ctx.build_ms, 1), "host_bvh_ms": round(host_build_ms, 1)}
+    for k, v in PARAMS.items():
+        ctx.set_param(k, v)
 
     for _ in range(args.warmup):
         ctx.render(W, H, A, download=False)
@@ -354,6 +369,8 @@ def main():
         "kernel_ms_per_step": {k: round(tot[k] / args.steps, 3) for k in ("primary_ms", "trace_ms", "shadow_ms",
                                                                         "shade_ms")},
     }
+    if PARAMS:
+        line["params"] = PARAMS
     line["scene_setup"] = setup
     ctx.close()
     if not args.no_analytic:

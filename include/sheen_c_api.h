@@ -235,7 +235,13 @@ int sheen_context_set_chunk(SheenContext* ctx, int samples_per_chunk);
  * one run; the image is the same either way); "shadow_order" (default 1: a
  * shading block queues its shadow requests that leave the surface before
  * those that enter it) and "sec_order" (default 1: its reflection rays
- * before its refraction rays), both only reorder queues; "shade_block"
+ * before its refraction rays), both only reorder queues; "sec_sort" (0:
+ * off; 1: before each trace round the secondary rays are sorted into 16
+ * bins, reflection or refraction times the direction's octant; 2: 48 bins,
+ * also by the direction's dominant axis; a stable device sort, so a wave
+ * traces rays of one kind heading one way; the image is the same either
+ * way; default 2 for scenes of spheres, triangles and planes, 0 otherwise);
+ * "shade_block"
  * (workgroup size of the shading kernels, a multiple of 64 up to 1024; 0 =
  * the compiled bound, 1024 for scenes of spheres, triangles and planes, 512
  * otherwise).  Returns 0, or -1 for an unknown name or an invalid value. */
@@ -262,7 +268,8 @@ void sheen_batch_destroy(SheenBatch* batch);
  * kernels directly instead of replaying the captured graph) and
  * "scene_table" (1..512, default 512: entries of the per-block LDS copy of
  * the scenes' first slots that the kernels search to find a slot's scene;
- * beyond that many scenes the search ends in global memory). */
+ * beyond that many scenes the search ends in global memory); "sec_sort" is
+ * also set on every scene's own context (sheen_batch_render_serial). */
 int sheen_batch_set_param(SheenBatch* batch, const char* name, int value);
 /* One frame of every scene.  stats: ray counts of the whole batch, frame_ms
  * (GPU time, hipEvents) and trace_ms (wall clock of the call). */

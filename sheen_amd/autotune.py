@@ -14,7 +14,8 @@ queue depth, rays per thread; `render_kernel.cu:1240-1262` launches one
   (the LDS stack columns are laid out per 256-lane block), as is the BVH
   width, 8 children (one 128-byte node);
 * queue order     -> `shadow_order`, `sec_order`, `light_runs` (which rays
-  share a wave of the trace kernels).
+  share a wave of the trace kernels) and `sec_sort` (secondary rays sorted
+  by kind and direction before each trace round).
 
 `tune()` runs coordinate descent: each knob is swept with the others held,
 scored by the GPU time of the kernel it shapes (hipEvents on the render
@@ -35,11 +36,13 @@ SPACE = {
     "light_runs": (1, 0),
     "shadow_order": (1, 0),
     "sec_order": (1, 0),
+    "sec_sort": (0, 1, 2),
 }
 
 # the library's built-in launch shapes (tuned on the 1M-triangle 1080p frame)
 DEFAULTS = {"primary_grid": 64, "sec_grid": 6, "shadow_grid": 64, "shade_grid": 1, "shade_sec_grid": 1, "chunk": 0,
-            "shade_block": 0, "overlap": 0, "light_runs": 1, "shadow_order": 1, "sec_order": 1}
+            "shade_block": 0, "overlap": 0, "light_runs": 1, "shadow_order": 1, "sec_order": 1,
+            "sec_sort": 2}
 
 
 def _score(name, st):
@@ -129,6 +132,7 @@ BATCH_SPACE = {
     "shade_block": (1024, 512, 256),
     "shadow_order": (1, 0),
     "sec_order": (1, 0),
+    "sec_sort": (0, 1, 2),
 }
 
 

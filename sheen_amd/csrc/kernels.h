@@ -82,6 +82,7 @@ struct SampleBufs {
 #define SF_REFR 8
 #define SF_EV2 16
 
+#define SORT_BINS_MAX 64
 // Secondary ray queue entry (reflection or refraction face), 32 bytes.
 struct SecRay {
     float ox, oy, oz, w;
@@ -109,6 +110,7 @@ struct Counters {
     uint32_t sec_n[4];       // secondary queues (ping-pong) sizes
     uint32_t cont_n[2];      // shadow continuations queued for the second and third pass
     uint32_t pad[26];        // (the shadow queue counts are per light: EventBufs::shadow_nl)
+    uint32_t sort_tot[SORT_BINS_MAX];  // per-bin totals of the secondary-ray sort (k_sec_scan)
     CtrSum shadow_passes;
     CtrSum nodes, prims, snodes, sprims, pnodes, pprims;
     CtrSum reflection, refraction, extra_shadow;
@@ -810,6 +812,144 @@ __global__ TRACE_BOUNDS void k_shadow_cont(SRC src, EventBufs EB, Counters* C, i
         wave_add64(C->sprims, ts.prims);
         wave_add64(C->sspills, st.spills);
         wave_add64(C->switers, ts.witers);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Secondary-ray sorting.  Before a round's trace, the queue is reordered by a
+// bin key so that a wave holds rays of one kind (reflection or refraction)
+// travelling into one cell of the direction sphere: mode 1, the 8 octants
+// (16 bins); mode 2, also the dominant axis (24 cells, 64 bins, 48 used).
+// The order inside a bin is the queue's own (stable), which is tile-major,
+// so a wave's rays also start close together.  Every entry carries its slot,
+// so the order changes no result.  Three passes over tiles of SORT_TILE
+// entries: per-tile bin counts (k_sec_hist), an exclusive scan of each bin's
+// counts over the tiles with the bin totals (k_sec_scan, one block per bin),
+// and a stable scatter that ranks each tile's entries with wave ballots
+// (k_sec_scatter).  The queue length is read on the device; the grids are
+// sized on the queue's capacity and loop over the tiles in use.
+#define SORT_BLOCK 256
+#define SORT_ITEMS 4
+#define SORT_TILE (SORT_BLOCK * SORT_ITEMS)
+#define SORT_BINS SORT_BINS_MAX
+#define SORT_BIN_BITS 6
+
+SHD uint32_t sec_sort_key(const SecRay& e, int mode) {
+    const uint32_t kind = (e.meta >> 28) & 1u;
+    const uint32_t oct = (e.dx < 0.0f ? 4u : 0u) | (e.dy < 0.0f ? 2u : 0u) | (e.dz < 0.0f ? 1u : 0u);
+    if (mode == 1) return (kind << 3) | oct;
+    const float ax = fabsf(e.dx), ay = fabsf(e.dy), az = fabsf(e.dz);
+    const uint32_t major = (ax >= ay && ax >= az) ? 0u : (ay >= az ? 1u : 2u);
+    return (kind << 5) | (major << 3) | oct;
+}
+// Tiles in use for a queue of n entries.
+SHD uint32_t sort_tiles(uint32_t n) { return (n + SORT_TILE - 1) / SORT_TILE; }
+
+// hist[bin * max_tiles + tile] = entries of `tile` in `bin`.
+__global__ __launch_bounds__(SORT_BLOCK) void k_sec_hist(const SecRay* q, const Counters* C, int which, int mode,
+                                                         uint32_t* hist, uint32_t max_tiles) {
+    __shared__ uint32_t h[SORT_BINS];
+    const uint32_t n = C->sec_n[which];
+    const uint32_t T = sort_tiles(n);
+    for (uint32_t tile = blockIdx.x; tile < T; tile += gridDim.x) {
+        for (uint32_t b = threadIdx.x; b < SORT_BINS; b += blockDim.x) h[b] = 0u;
+        __syncthreads();
+        for (int k = 0; k < SORT_ITEMS; k++) {
+            const uint32_t j = tile * SORT_TILE + (uint32_t)k * SORT_BLOCK + threadIdx.x;
+            if (j < n) atomicAdd(&h[sec_sort_key(q[j], mode)], 1u);
+        }
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < SORT_BINS; b += blockDim.x) hist[b * max_tiles + tile] = h[b];
+        __syncthreads();
+    }
+}
+
+// Block b: exclusive scan of bin b's tile counts in place; the bin's total
+// to tot[b].
+__global__ __launch_bounds__(1024) void k_sec_scan(const Counters* C, int which, uint32_t* hist, uint32_t max_tiles,
+                                                   uint32_t* tot) {
+    __shared__ uint32_t part[1024 / 64];
+    __shared__ uint32_t carry;
+    const uint32_t T = sort_tiles(C->sec_n[which]);
+    uint32_t* row = hist + (size_t)blockIdx.x * max_tiles;
+    if (threadIdx.x == 0) carry = 0u;
+    __syncthreads();
+    for (uint32_t t0 = 0; t0 < T; t0 += blockDim.x) {
+        const uint32_t t = t0 + threadIdx.x;
+        const uint32_t v = t < T ? row[t] : 0u;
+        // inclusive scan within the wave, then across the block's waves
+        uint32_t x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if ((int)__lane_id() >= off) x += y;
+        }
+        const uint32_t w = threadIdx.x >> 6;
+        if (__lane_id() == 63u) part[w] = x;
+        __syncthreads();
+        uint32_t before = carry;
+        for (uint32_t k = 0; k < w; k++) before += part[k];
+        if (t < T) row[t] = before + x - v;
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) carry = before + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+// Stable scatter of the queue into `out` in bin order.
+__global__ __launch_bounds__(SORT_BLOCK) void k_sec_scatter(const SecRay* q, const Counters* C, int which, int mode,
+                                                            const uint32_t* hist, uint32_t max_tiles,
+                                                            const uint32_t* tot, SecRay* out) {
+    __shared__ uint32_t start[SORT_BINS];      // bin start + this tile's offset in the bin
+    __shared__ uint32_t wc[SORT_BLOCK / 64][SORT_BINS];
+    __shared__ uint32_t pre[SORT_BLOCK / 64][SORT_BINS];
+    const uint32_t n = C->sec_n[which];
+    const uint32_t T = sort_tiles(n);
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+    for (uint32_t tile = blockIdx.x; tile < T; tile += gridDim.x) {
+        static_assert(SORT_BINS == 64, "one wave scans the bin totals");
+        if (threadIdx.x < SORT_BINS) {  // wave 0: exclusive scan of the totals, plus this tile's offsets
+            const uint32_t t = tot[threadIdx.x];
+            uint32_t x = t;
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t y = __shfl_up(x, off);
+                if ((int)__lane_id() >= off) x += y;
+            }
+            start[threadIdx.x] = x - t + hist[(size_t)threadIdx.x * max_tiles + tile];
+        }
+        for (int k = 0; k < SORT_ITEMS; k++) {
+            const uint32_t j = tile * SORT_TILE + (uint32_t)k * SORT_BLOCK + threadIdx.x;
+            const bool valid = j < n;
+            SecRay e;
+            uint32_t key = SORT_BINS;  // invalid lanes match no valid lane's key
+            if (valid) {
+                e = q[j];
+                key = sec_sort_key(e, mode);
+            }
+            // lanes of the wave with the same key, by ballots over its bits
+            uint64_t same = __ballot(valid);
+            if (!valid) same = ~same;
+            for (int bit = 0; bit < SORT_BIN_BITS; bit++) {
+                const uint64_t ones = __ballot((key >> bit) & 1u);
+                same &= ((key >> bit) & 1u) ? ones : ~ones;
+            }
+            for (uint32_t b = threadIdx.x; b < (SORT_BLOCK / 64) * SORT_BINS; b += blockDim.x) (&wc[0][0])[b] = 0u;
+            __syncthreads();
+            if (valid && (same & lt) == 0ull) wc[wave][key] = (uint32_t)__popcll(same);
+            __syncthreads();
+            for (uint32_t b = threadIdx.x; b < SORT_BINS; b += blockDim.x) {
+                uint32_t acc = start[b];
+                for (int w = 0; w < SORT_BLOCK / 64; w++) {
+                    pre[w][b] = acc;
+                    acc += wc[w][b];
+                }
+                start[b] = acc;
+            }
+            __syncthreads();
+            if (valid) out[pre[wave][key] + (uint32_t)__popcll(same & lt)] = e;
+        }
+        __syncthreads();
     }
 }
 

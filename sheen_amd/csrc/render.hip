@@ -61,12 +61,13 @@ struct WorkBufs {
     uint32_t* shadow_nl = nullptr;  // per-light request counts and their snapshot
     SecRay* sec_q[2] = {nullptr, nullptr};
     int4* sec_hits = nullptr;
+    uint32_t* sort_hist = nullptr;  // secondary-ray sort: per-bin counts of each queue tile
     float4* beam_add = nullptr;
     float4* media_p = nullptr;
     float4* media_r = nullptr;
     void release() {
         void* ps[] = {hit_p, samp, ev_base, ev_caustic, contrib, trans, shadow_q, cont[0], cont[1], shadow_nl, sec_q[0],
-                      sec_q[1], sec_hits, beam_add, media_p, media_r};
+                      sec_q[1], sec_hits, sort_hist, beam_add, media_p, media_r};
         for (void* p : ps) if (p) (void)hipFree(p);
         *this = WorkBufs();
     }
@@ -118,6 +119,9 @@ struct SheenContext {
     bool light_runs = true;
     // queue order of the shading kernels (EventBufs::order bits)
     int order = ORDER_SHADOW | ORDER_SEC;
+    // secondary rays sorted before each trace round: 0 off, 1 kind + octant,
+    // 2 kind + octant + dominant axis (sec_sort_key); set in context_create
+    int sec_sort = 0;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     struct Mark { int slot; hipEvent_t a, b; };
@@ -509,6 +513,10 @@ static SheenContext* context_create(const SheenScene* in, int flags = 0) {
     d.any_strands = d.count[C_STRAND] > 0;
     c->full = d.count[C_STRAND] || d.count[C_HF] || d.count[C_SE] || d.count[C_LATHE] || d.count[C_PRISM] ||
               d.csg_root >= 0 || d.num_media > 0 || c->n_beam > 0;
+    // sorted secondary rays: 0.8 % faster on the 1M-triangle frame, 3 %
+    // slower on the analytic 10k frame (whose shading favours rays that
+    // start close together over rays that head one way)
+    c->sec_sort = c->full ? 0 : 2;
     // the differentiable path covers primary hits on spheres, triangles and
     // planes under direct light
     for (int i = 0; i < in->num_materials && c->adj_reason.empty(); i++) {
@@ -535,10 +543,13 @@ static SheenContext* context_create(const SheenScene* in, int flags = 0) {
 // quarter of them (requests beyond it finish their passes in the first
 // shadow kernel).
 static size_t cont_entries(size_t shadow_entries) { return std::max<size_t>(64, shadow_entries / 4); }
+// Tiles of a secondary queue of 2 * slots entries, and the sort's count words.
+static uint32_t sort_max_tiles(size_t slots) { return (uint32_t)((2 * slots + SORT_TILE - 1) / SORT_TILE); }
+static size_t sort_hist_words(size_t slots) { return (size_t)SORT_BINS * sort_max_tiles(slots); }
 
 static size_t slot_bytes(int L, int n_beam, bool caustic, bool media) {
     size_t b = 16 /*hit*/ + 16 /*samp*/ + 3 * 16 /*base*/ + (caustic ? 3 * 16 : 0) + 3 * (size_t)L * (16 + 4) +
-               2 * 32 /*sec queues*/ + 2 * 16 /*sec hits*/ + 3 * (size_t)L * 32 /*shadow queue*/ +
+               4 * 32 /*sec queues*/ + 2 * 16 /*sec hits*/ + 1 /*sort counts*/ + 3 * (size_t)L * 32 /*shadow queue*/ +
                2 * 3 * (size_t)L * 32 / 4 /*shadow continuations*/ + (size_t)n_beam * 16 + (media ? 32 : 0);
     return b;
 }
@@ -566,7 +577,8 @@ static bool ensure_bufs(WorkBufs& w, size_t slots, int L, int n_beam, bool caust
               alloc_dev((void**)&w.shadow_nl, 2 * Lq * sizeof(uint32_t)) &&
               alloc_dev((void**)&w.sec_q[0], 2 * slots * sizeof(SecRay)) &&
               alloc_dev((void**)&w.sec_q[1], 2 * slots * sizeof(SecRay)) &&
-              alloc_dev((void**)&w.sec_hits, 2 * slots * sizeof(int4));
+              alloc_dev((void**)&w.sec_hits, 2 * slots * sizeof(int4)) &&
+              alloc_dev((void**)&w.sort_hist, sort_hist_words(slots) * sizeof(uint32_t));
     if (ok && caustic) ok = alloc_dev((void**)&w.ev_caustic, ev * sizeof(float4));
     if (ok && n_beam > 0) ok = alloc_dev((void**)&w.beam_add, slots * n_beam * sizeof(float4));
     if (ok && media) ok = alloc_dev((void**)&w.media_p, slots * sizeof(float4)) && alloc_dev((void**)&w.media_r, slots * sizeof(float4));
@@ -611,6 +623,17 @@ struct AdjCall {
     const float* grad;
     AdjGrads G;
 };
+
+// Sorts secondary queue `in` (count C->sec_n[which]) into `out` by
+// sec_sort_key (mode 1 or 2); cap_slots sizes the grids and the counts.
+static void launch_sec_sort(hipStream_t s, int num_cu, const SecRay* in, SecRay* out, Counters* ctr, int which,
+                            int mode, uint32_t* hist, size_t cap_slots) {
+    const uint32_t max_tiles = sort_max_tiles(cap_slots);
+    const int g = (int)std::min<uint32_t>(max_tiles, (uint32_t)num_cu * 8u);
+    k_sec_hist<<<g, SORT_BLOCK, 0, s>>>(in, ctr, which, mode, hist, max_tiles);
+    k_sec_scan<<<SORT_BINS, 1024, 0, s>>>(ctr, which, hist, max_tiles, ctr->sort_tot);
+    k_sec_scatter<<<g, SORT_BLOCK, 0, s>>>(in, ctr, which, mode, hist, max_tiles, ctr->sort_tot, out);
+}
 
 // The second and third shadow passes of the requests that went through
 // glass (scenes of spheres, triangles and planes; the others keep every pass
@@ -679,9 +702,17 @@ static void run_chunk(SheenContext* c, WorkBufs& w, Counters* ctr, float4* frame
         if (overlap) (void)hipEventRecord(c->ev_side[1], c->side);
         // three rounds: reflection + first refraction face, then faces 2 and 3
         for (int round = 0; round < 3; round++) {
-            const int in_q = round & 1, out_q = (round + 1) & 1;
+            // unsorted: queues 0 and 1 alternate; sorted: a round's queue is
+            // sorted from 0 into 1, traced and shaded from 1, and the next
+            // faces queue into 0 again
+            const bool sorted = c->sec_sort > 0;
+            const int in_q = sorted ? 1 : (round & 1), out_q = sorted ? 0 : ((round + 1) & 1);
             const int in_ctr = round, out_ctr = round + 1;
             const size_t cap = 2 * (size_t)F.n_slots;
+            if (sorted) {
+                Timer t(c, 1);
+                launch_sec_sort(s, c->num_cu, w.sec_q[0], w.sec_q[1], ctr, in_ctr, c->sec_sort, w.sort_hist, w.cap_slots);
+            }
             {
                 Timer t(c, 1);
                 if (c->counting)
@@ -903,6 +934,7 @@ static size_t carve_work(unsigned char* base, size_t slots, int L, int n_beam, b
     w.sec_q[0] = (SecRay*)take(2 * slots * sizeof(SecRay));
     w.sec_q[1] = (SecRay*)take(2 * slots * sizeof(SecRay));
     w.sec_hits = (int4*)take(2 * slots * sizeof(int4));
+    w.sort_hist = (uint32_t*)take(sort_hist_words(slots) * sizeof(uint32_t));
     if (caustic) w.ev_caustic = (float4*)take(ev * sizeof(float4));
     if (n_beam > 0) w.beam_add = (float4*)take(slots * n_beam * sizeof(float4));
     if (media) {
@@ -1075,6 +1107,10 @@ int sheen_context_set_param(SheenContext* ctx, const char* name, int value) {
     if (value < 0) { sheen_set_error("negative parameter value"); return -1; }
     if (n == "overlap") ctx->overlap = value != 0;
     else if (n == "light_runs") ctx->light_runs = value != 0;
+    else if (n == "sec_sort") {
+        if (value < 0 || value > 2) { sheen_set_error("sec_sort must be 0, 1 or 2"); return -1; }
+        ctx->sec_sort = (int)value;
+    }
     else if (n == "shadow_order") ctx->order = value ? (ctx->order | ORDER_SHADOW) : (ctx->order & ~ORDER_SHADOW);
     else if (n == "sec_order") ctx->order = value ? (ctx->order | ORDER_SEC) : (ctx->order & ~ORDER_SEC);
     else if (n == "primary_grid") ctx->primary_grid = value;
@@ -1333,6 +1369,7 @@ struct SheenBatch {
     int n_beam = 0;
     int order = ORDER_SHADOW | ORDER_SEC;  // as SheenContext::order
     int shade_block = 0;                   // as SheenContext::shade_block
+    int sec_sort = 0;                      // as SheenContext::sec_sort
     size_t chunk_slots = 0;
     bool use_graph = true, dirty = true;
     int scene_table = SCENE_LDS;  // LDS entries of the slot -> scene search
@@ -1418,7 +1455,9 @@ static void batch_enqueue_t(SheenBatch* b) {
         k_primary<false, FULL><<<grid_n(cu, n, b->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, b->ctr);
         k_shade_primary<FULL><<<grid_n(cu, n, b->shade_grid, SB_), SB_, 0, s>>>(src, SB, EB, w.sec_q[0], b->ctr);
         for (int round = 0; round < 3; round++) {
-            const int in_q = round & 1, out_q = (round + 1) & 1;
+            const bool sorted = b->sec_sort > 0;  // as in run_chunk
+            const int in_q = sorted ? 1 : (round & 1), out_q = sorted ? 0 : ((round + 1) & 1);
+            if (sorted) launch_sec_sort(s, cu, w.sec_q[0], w.sec_q[1], b->ctr, round, b->sec_sort, w.sort_hist, w.cap_slots);
             k_trace_sec<false, FULL><<<grid_n(cu, 2 * n, b->sec_grid), TRACE_BLOCK, 0, s>>>(src, w.sec_q[in_q], round,
                                                                                            w.sec_hits, b->ctr);
             k_shade_sec<FULL><<<grid_n(cu, 2 * n, b->shade_sec_grid, SB_), SB_, 0, s>>>(
@@ -1502,6 +1541,9 @@ SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* width
     HIP_OR(hipMemcpy(b->d_scenes1, table1.data(), sizeof(DScene) * count, hipMemcpyHostToDevice), nullptr);
     HIP_OR(hipMemcpy(b->d_frames, b->frames.data(), sizeof(FrameDesc) * count, hipMemcpyHostToDevice), nullptr);
     HIP_OR(hipMemcpy(b->d_base, b->base.data(), sizeof(uint32_t) * (count + 1), hipMemcpyHostToDevice), nullptr);
+    // as a context's default; the scenes' own renders sort alike
+    b->sec_sort = b->full ? 0 : 2;
+    for (SheenContext* c : b->ctx) c->sec_sort = b->sec_sort;
     b->chunk_slots = batch_default_chunk(b.get());
     b->chunk_slots = std::min<size_t>(b->chunk_slots, ((size_t)slot0 + 63) / 64 * 64 + 64 * 64);
     batch_chunks(b.get(), b->chunk_slots);
@@ -1527,6 +1569,11 @@ int sheen_batch_set_param(SheenBatch* b, const char* name, int value) {
     else if (n == "graph") b->use_graph = value != 0;
     else if (n == "shadow_order") b->order = value ? (b->order | ORDER_SHADOW) : (b->order & ~ORDER_SHADOW);
     else if (n == "sec_order") b->order = value ? (b->order | ORDER_SEC) : (b->order & ~ORDER_SEC);
+    else if (n == "sec_sort") {
+        if (value < 0 || value > 2) { sheen_set_error("sec_sort must be 0, 1 or 2"); return -1; }
+        b->sec_sort = (int)value;
+        for (SheenContext* c : b->ctx) c->sec_sort = (int)value;  // the serial renders sort alike
+    }
     else if (n == "scene_table") {
         if (value < 1 || value > SCENE_LDS) { sheen_set_error("scene_table must be 1..%d", SCENE_LDS); return -1; }
         b->scene_table = value;

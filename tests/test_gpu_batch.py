@@ -73,6 +73,23 @@ def test_chunks_across_scene_boundaries(batch):
     b.set_param("chunk", 0)
 
 
+def test_sorted_secondary_rays_in_a_batch(batch):
+    """Sorting each round's secondary rays (their scenes mixed in one queue)
+    changes no batched image, with chunks inside and across scenes."""
+    b = batch["b"]
+    b.set_param("sec_sort", 0)
+    b.render()
+    ref = [b.image(k) for k in range(b.n)]
+    for mode, chunk in ((1, 0), (2, 0), (2, 64 * 8 * 7)):
+        b.set_param("sec_sort", mode)
+        b.set_param("chunk", chunk)
+        b.render()
+        for k in range(b.n):
+            assert np.array_equal(b.image(k), ref[k]), (mode, chunk, k)
+    b.set_param("sec_sort", 0)
+    b.set_param("chunk", 0)
+
+
 def test_scene_table_sizes(batch):
     """The slot -> scene search gives the same images whether the scenes'
     first slots all sit in the LDS table or only every step-th of them."""

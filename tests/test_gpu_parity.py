@@ -133,6 +133,32 @@ def test_side_stream_shadows_do_not_change_the_image(cfg):
             assert st[k] == st0[k], (key, k)
 
 
+@pytest.mark.parametrize("cfg", [2, 101, 103, 104])
+def test_sorted_secondary_rays_do_not_change_the_image(cfg):
+    """Secondary rays sorted by kind and direction before each trace round
+    (sec_sort 1: octant, 2: octant and dominant axis) give the same image and
+    ray counts as the queue in shading order, within one chunk and across
+    chunk boundaries, with the side-stream shadows on and off."""
+    from sheen_amd import scenes, sheen
+    sc = scenes.generate(cfg, **({"detail": 40} if cfg == 2 else {}))
+    out = {}
+    with sheen.Context(sc.scene) as ctx:
+        for mode in (0, 1, 2):
+            ctx.set_param("sec_sort", mode)
+            for overlap, chunk in ((0, 0), (1, 64 * 4 * 5)):
+                ctx.set_param("overlap", overlap)
+                ctx.set_chunk(chunk)
+                out[mode, overlap, chunk] = ctx.render(160, 120, 4)
+        with pytest.raises(RuntimeError, match="sec_sort"):
+            ctx.set_param("sec_sort", 3)
+    ref, st0 = out[0, 0, 0]
+    assert st0["reflection_rays"] + st0["refraction_rays"] > 0
+    for key, (img, st) in out.items():
+        assert np.array_equal(img, ref), key
+        for k in ("shadow_rays", "reflection_rays", "refraction_rays", "total_rays"):
+            assert st[k] == st0[k], (key, k)
+
+
 def test_autotuner_picks_from_its_space_and_keeps_the_image():
     from sheen_amd import scenes, sheen, autotune
     sc = scenes.generate(2, detail=40)
