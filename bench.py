@@ -299,12 +299,14 @@ def main():
 
     ctx.set_profiling(True)
     tot = {"rays": 0, "frame_ms": 0.0, "primary_ms": 0.0, "trace_ms": 0.0, "shadow_ms": 0.0, "shade_ms": 0.0,
+           "shadow_cont_ms": 0.0,
            "shadow_launches": 0, "trace_launches": 0}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         _, st = ctx.render(W, H, A, download=False)
         tot["rays"] += st["total_rays"]
-        for k in ("frame_ms", "primary_ms", "trace_ms", "shadow_ms", "shade_ms", "shadow_launches", "trace_launches"):
+        for k in ("frame_ms", "primary_ms", "trace_ms", "shadow_ms", "shade_ms", "shadow_cont_ms", "shadow_launches",
+                  "trace_launches"):
             tot[k] += st[k]
     t1 = time.perf_counter()
     ctx.set_profiling(False)
@@ -367,7 +369,7 @@ def main():
                     "the unit that bounds the kernel",
         },
         "kernel_ms_per_step": {k: round(tot[k] / args.steps, 3) for k in ("primary_ms", "trace_ms", "shadow_ms",
-                                                                        "shade_ms")},
+                                                                        "shadow_cont_ms", "shade_ms")},
     }
     if PARAMS:
         line["params"] = PARAMS

@@ -154,6 +154,14 @@ typedef struct SheenRayStats {
      * kernel's waves; shadow_node_fetches / shadow_visit_lanes is the share
      * of wave lanes that visit a node in the traversal loop. */
     uint64_t shadow_visit_lanes;
+    /* The shadow_* counts and shadow_ms cover the shadow kernel (every
+     * request's first pass); the second and third passes of requests that
+     * went through glass run in continuation kernels of their own (scenes of
+     * spheres, triangles and planes), counted here: */
+    uint64_t cont_node_fetches;
+    uint64_t cont_prim_tests;
+    float shadow_cont_ms;
+    int cont_launches;
 } SheenRayStats;
 
 typedef struct SheenContext SheenContext;

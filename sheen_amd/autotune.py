@@ -51,8 +51,8 @@ def _score(name, st):
         return st["primary_ms"]
     if name == "sec_grid":
         return st["trace_ms"] - st["primary_ms"]
-    if name == "shadow_grid":
-        return st["shadow_ms"]
+    if name == "shadow_grid":  # also the grid of the glass continuation kernels
+        return st["shadow_ms"] + st["shadow_cont_ms"]
     if name == "shade_grid":
         return st["shade_primary_ms"]
     if name == "shade_sec_grid":

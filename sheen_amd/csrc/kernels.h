@@ -116,6 +116,7 @@ struct Counters {
     CtrSum reflection, refraction, extra_shadow;
     CtrSum spills, sspills;
     CtrSum switers;  // node-visit iterations of the shadow kernel's waves
+    CtrSum cnodes, cprims;  // the shadow continuation kernels
 };
 
 SHD uint32_t wave_sum(uint32_t v) {
@@ -808,10 +809,9 @@ __global__ TRACE_BOUNDS void k_shadow_cont(SRC src, EventBufs EB, Counters* C, i
     }
     wave_add64(C->shadow_passes, passes);
     if (COUNT) {
-        wave_add64(C->snodes, ts.nodes);
-        wave_add64(C->sprims, ts.prims);
+        wave_add64(C->cnodes, ts.nodes);
+        wave_add64(C->cprims, ts.prims);
         wave_add64(C->sspills, st.spills);
-        wave_add64(C->switers, ts.witers);
     }
 }
 

@@ -82,6 +82,10 @@ struct SheenContext {
     HostSceneInfo info;
     int n_beam = 0;
     bool full = true;  // scene uses a kind beyond spheres, triangles and planes
+    // some material reflects or transmits (secondary rays can exist), some
+    // transmits above 0.5 (a shadow ray can pass glass): the rounds and
+    // kernels for them are launched only then
+    bool any_secondary = true, any_glass = true;
     WorkBufs wb;
     WorkBufs* wbp = &wb;  // the work buffers in use (a batch shares one set among its scenes)
     float4* frame = nullptr;
@@ -519,6 +523,12 @@ static SheenContext* context_create(const SheenScene* in, int flags = 0) {
     c->sec_sort = c->full ? 0 : 2;
     // the differentiable path covers primary hits on spheres, triangles and
     // planes under direct light
+    c->any_secondary = c->any_glass = false;
+    for (int i = 0; i < in->num_materials; i++) {
+        const Material& m = in->materials[i];
+        c->any_secondary = c->any_secondary || m.reflect > 0.0f || m.transmission > 0.0f;
+        c->any_glass = c->any_glass || m.transmission > 0.5f;
+    }
     for (int i = 0; i < in->num_materials && c->adj_reason.empty(); i++) {
         const Material& m = in->materials[i];
         if (m.reflect > 0.0f) c->adj_reason = "material with reflection";
@@ -677,7 +687,7 @@ static void run_chunk(SheenContext* c, WorkBufs& w, Counters* ctr, float4* frame
             // (differentiable scenes have no secondary rays)
             const size_t cap = 3 * (size_t)F.n_slots * std::max(1, c->ds.num_lights);
             k_shadow<false, FULL><<<grid_for(c, cap, c->shadow_grid), TRACE_BLOCK, 0, s>>>(src, EB, ctr, 0);
-            launch_shadow_cont<FULL>(s, grid_for(c, EB.cont_cap, c->shadow_grid), false, src, EB, ctr);
+            if (c->any_glass) launch_shadow_cont<FULL>(s, grid_for(c, EB.cont_cap, c->shadow_grid), false, src, EB, ctr);
             S.shadow_launches++;
             // one block per 8x8 tile of samples at a time
             const size_t tiles = F.n_slots / (64u * (uint32_t)ns);
@@ -701,7 +711,7 @@ static void run_chunk(SheenContext* c, WorkBufs& w, Counters* ctr, float4* frame
         }
         if (overlap) (void)hipEventRecord(c->ev_side[1], c->side);
         // three rounds: reflection + first refraction face, then faces 2 and 3
-        for (int round = 0; round < 3; round++) {
+        for (int round = 0; round < (c->any_secondary ? 3 : 0); round++) {
             // unsorted: queues 0 and 1 alternate; sorted: a round's queue is
             // sorted from 0 into 1, traced and shaded from 1, and the next
             // faces queue into 0 again
@@ -737,10 +747,11 @@ static void run_chunk(SheenContext* c, WorkBufs& w, Counters* ctr, float4* frame
             S.shadow_launches++;
         }
         if (overlap) (void)hipStreamWaitEvent(s, c->ev_side[1], 0);
-        {
+        if (!FULL && EB.cont_cap && c->any_glass) {
             // the continuations of both shadow launches (one queue)
-            Timer t(c, 2);
+            Timer t(c, 5);
             launch_shadow_cont<FULL>(s, grid_for(c, EB.cont_cap, c->shadow_grid), c->counting, src, EB, ctr);
+            S.cont_launches += 2;
         }
         {
             Timer t(c, 3);
@@ -842,6 +853,7 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
         if (m.slot == 0) { S.primary_ms += ms; S.trace_ms += ms; }
         else if (m.slot == 1) S.trace_ms += ms;
         else if (m.slot == 2) S.shadow_ms += ms;
+        else if (m.slot == 5) S.shadow_cont_ms += ms;
         else if (m.slot == 4) { S.shade_ms += ms; S.shade_primary_ms += ms; }
         else S.shade_ms += ms;
     }
@@ -868,6 +880,8 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     S.shadow_node_fetches = ctr_total(host.snodes);
     S.shadow_prim_tests = ctr_total(host.sprims);
     S.shadow_visit_lanes = ctr_total(host.switers) * 64ull;
+    S.cont_node_fetches = ctr_total(host.cnodes);
+    S.cont_prim_tests = ctr_total(host.cprims);
     c->last = S;
     if (st) *st = S;
     return 0;
@@ -1366,6 +1380,7 @@ struct SheenBatch {
     // or beams (the pipeline then runs the instantiations for every kind);
     // caustics, media and the largest beam count size the shared buffers
     bool full = false, caustic = false, media = false;
+    bool any_secondary = false, any_glass = false;  // of any scene (SheenContext)
     int n_beam = 0;
     int order = ORDER_SHADOW | ORDER_SEC;  // as SheenContext::order
     int shade_block = 0;                   // as SheenContext::shade_block
@@ -1454,7 +1469,7 @@ static void batch_enqueue_t(SheenBatch* b) {
         k_reset<<<1, 64, 0, s>>>(b->ctr, EB.shadow_nl, EB.runs);
         k_primary<false, FULL><<<grid_n(cu, n, b->primary_grid), TRACE_BLOCK, 0, s>>>(src, w.hit_p, b->ctr);
         k_shade_primary<FULL><<<grid_n(cu, n, b->shade_grid, SB_), SB_, 0, s>>>(src, SB, EB, w.sec_q[0], b->ctr);
-        for (int round = 0; round < 3; round++) {
+        for (int round = 0; round < (b->any_secondary ? 3 : 0); round++) {
             const bool sorted = b->sec_sort > 0;  // as in run_chunk
             const int in_q = sorted ? 1 : (round & 1), out_q = sorted ? 0 : ((round + 1) & 1);
             if (sorted) launch_sec_sort(s, cu, w.sec_q[0], w.sec_q[1], b->ctr, round, b->sec_sort, w.sort_hist, w.cap_slots);
@@ -1464,7 +1479,7 @@ static void batch_enqueue_t(SheenBatch* b) {
                 src, SB, EB, w.sec_q[in_q], round, w.sec_hits, w.sec_q[out_q], round + 1, b->ctr);
         }
         k_shadow<false, FULL><<<grid_n(cu, 3 * n * (size_t)b->L, b->shadow_grid), TRACE_BLOCK, 0, s>>>(src, EB, b->ctr, 0);
-        launch_shadow_cont<FULL>(s, grid_n(cu, EB.cont_cap, b->shadow_grid), false, src, EB, b->ctr);
+        if (b->any_glass) launch_shadow_cont<FULL>(s, grid_n(cu, EB.cont_cap, b->shadow_grid), false, src, EB, b->ctr);
         k_accumulate<<<grid_n(cu, n, 8), 256, 0, s>>>(src, SB, EB, b->frame);
     }
 }
@@ -1502,6 +1517,8 @@ SheenBatch* sheen_batch_create(const SheenScene* const* scenes, const int* width
         if (!c) return nullptr;
         b->ctx.push_back(c);
         b->full = b->full || c->full;
+        b->any_secondary = b->any_secondary || c->any_secondary;
+        b->any_glass = b->any_glass || c->any_glass;
         b->caustic = b->caustic || (c->ds.pm.photons && c->ds.caustics > 0.0f);
         b->media = b->media || c->ds.num_media > 0;
         b->n_beam = std::max(b->n_beam, c->n_beam);

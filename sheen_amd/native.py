@@ -22,7 +22,8 @@ class SheenRayStats(C.Structure):
                                           "primary_prim_tests", "primary_stack_spills", "shadow_stack_spills")] + \
                [("frame_ms", C.c_float), ("trace_ms", C.c_float), ("primary_ms", C.c_float), ("shadow_ms", C.c_float), ("shade_ms", C.c_float),
                 ("trace_launches", C.c_int), ("shadow_launches", C.c_int), ("shade_primary_ms", C.c_float),
-                ("shadow_visit_lanes", C.c_uint64)]
+                ("shadow_visit_lanes", C.c_uint64), ("cont_node_fetches", C.c_uint64), ("cont_prim_tests", C.c_uint64),
+                ("shadow_cont_ms", C.c_float), ("cont_launches", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
