@@ -317,11 +317,12 @@ def main():
     # one shadow and one primary launch per frame chunk
     chunks = max(1, tot["shadow_launches"] // args.steps)
     kernels = {
-        "k_shadow": (tot["shadow_ms"], cst["shadow_node_fetches"] * NODE_BYTES + cst["shadow_prim_tests"] * TRI_LEAF_BYTES),
-        "k_primary": (tot["primary_ms"], cst["primary_node_fetches"] * NODE_BYTES + cst["primary_prim_tests"] * TRI_LEAF_BYTES),
+        "k_shadow": (tot["shadow_ms"], cst["shadow_node_fetches"], cst["shadow_prim_tests"]),
+        "k_primary": (tot["primary_ms"], cst["primary_node_fetches"], cst["primary_prim_tests"]),
     }
     dom = max(kernels, key=lambda k: kernels[k][0])
-    ms, frame_bytes = kernels[dom]
+    ms, frame_nodes, frame_prims = kernels[dom]
+    frame_bytes = frame_nodes * NODE_BYTES + frame_prims * TRI_LEAF_BYTES
     bytes_per_launch = frame_bytes / chunks
     avg_launch_ms = ms / (chunks * args.steps)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
@@ -356,6 +357,8 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "bytes_per_launch": int(bytes_per_launch),
+            "node_fetches_per_launch": frame_nodes // chunks,
+            "prim_tests_per_launch": frame_prims // chunks,
             # counting frame: share of a shadow-kernel wave's lanes that visit a
             # node per traversal iteration (the rest have finished their ray)
             "shadow_traversal_lane_use": round(cst["shadow_node_fetches"] / max(1, cst["shadow_visit_lanes"]), 4),
