@@ -321,21 +321,37 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
         // `share`: every lane traces the same scene (a batch's lanes may hold
         // rays of different scenes, whose leaf arrays differ)
         if (LEAF_BATCH && share) {
-            const uint64_t active = __ballot(1);
-            uint32_t n = 0;
-            for (uint32_t lm = leafmask; lm; lm &= lm - 1) n += ((uint32_t)(meta >> (8 * __builtin_ctz(lm))) & 0xFFu) >> 5;
             const uint32_t lane = __lane_id();
             const uint64_t lt = (1ull << lane) - 1ull;
-            uint32_t off = 0, N = 0;
+            // tests of the leaf slots in `take`, and their offset among the wave's
+            auto count = [&](uint32_t take, uint32_t& off, uint32_t& N) -> uint32_t {
+                uint32_t n = 0;
+                for (uint32_t lm = take; lm; lm &= lm - 1) n += ((uint32_t)(meta >> (8 * __builtin_ctz(lm))) & 0xFFu) >> 5;
+                off = 0;
+                N = 0;
 #pragma unroll
-            for (int b = 0; b < 6; b++) {
-                const uint64_t mb = __ballot((n >> b) & 1u);
-                // lanes below this one with bit b set (v_mbcnt)
-                off += __builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u)) << b;
-                N += (uint32_t)__popcll(mb) << b;
-            }
-            if (__ballot(n >= 2) != 0ull && N <= LEAF_LIST) {
-                for (uint32_t lm = leafmask, j = off; lm; lm &= lm - 1) {
+                for (int b = 0; b < 6; b++) {
+                    const uint64_t mb = __ballot((n >> b) & 1u);
+                    // lanes below this one with bit b set (v_mbcnt)
+                    off += __builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u)) << b;
+                    N += (uint32_t)__popcll(mb) << b;
+                }
+                return n;
+            };
+            // Rounds: every pending leaf at once when the wave's tests fit the
+            // list, else each lane's lowest pending leaf (a bottom node holds
+            // up to 8 leaves); single tests are left to the lanes themselves.
+            for (;;) {
+                const uint64_t active = __ballot(1);
+                uint32_t take = leafmask, off, N;
+                uint32_t n = count(take, off, N);
+                if (__ballot(n >= 2) == 0ull) break;
+                if (N > LEAF_LIST) {
+                    take = leafmask & (0u - leafmask);
+                    n = count(take, off, N);
+                    if (N > LEAF_LIST) break;
+                }
+                for (uint32_t lm = take, j = off; lm; lm &= lm - 1) {
                     const uint32_t m = (uint32_t)(meta >> (8 * __builtin_ctz(lm))) & 0xFFu;
                     for (uint32_t k = 0; k < (m >> 5); k++) st.list[j++] = (lane << LEAF_SLOT_BITS) | (prim_base + (m & 31u) + k);
                 }
@@ -364,13 +380,14 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
                 }
                 __builtin_amdgcn_wave_barrier();
                 const unsigned long long key = st.keys[lane];
+                leafmask &= ~take;
                 if (key < key0) {
                     T.ct = __uint_as_float((uint32_t)(key >> 32));
                     T.ci = (int)(uint32_t)key;
                     T.ch = true;
                     if (any) return true;
                 }
-                leafmask = 0;
+                if (__ballot(leafmask != 0u) == 0ull) break;
             }
         }
     }

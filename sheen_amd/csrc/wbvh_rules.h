@@ -184,32 +184,84 @@ WB_HD inline bool wb_quantize(WNode& wn, const WbBox* cb, int nc) {
     return true;
 }
 
-// One wide node's children from its binary node: the node itself when it is
-// a leaf, else its two children, then repeatedly the internal child of
-// largest area (first in slot order on ties) is replaced by its left child
-// and its right child appended, until 8 slots or only leaves.
+// Leaves of the binary subtree at i, left to right, into out[0..cap); returns
+// their number, or cap + 1 once the subtree is known to hold more than cap
+// (every pending subtree holds at least one leaf, so the stack never exceeds
+// cap entries).
+template <class NodeF>
+WB_HD inline int wb_treelet(int i, NodeF node, int* out, int cap) {
+    int stk[WBVH_WIDTH + 1];
+    int sp = 0, n = 0;
+    stk[sp++] = i;
+    while (sp > 0) {
+        const int v = stk[--sp];
+        WbBox box;
+        int l, r, cnt;
+        node(v, box, l, r, cnt);
+        if (cnt > 0) {
+            out[n++] = v;
+            continue;
+        }
+        if (n + sp + 2 > cap) return cap + 1;
+        stk[sp++] = r;
+        stk[sp++] = l;
+    }
+    return n;
+}
+
+// One wide node's children from its binary node.
+// * A leaf: the node itself.
+// * A subtree of at most 8 leaves: all of its leaves (left to right), so the
+//   bottom wide nodes are full instead of holding the two leaves of the
+//   binary tree's last level.
+// * Otherwise its two children; then repeatedly the child of largest area
+//   that holds more than 8 leaves is replaced by its left child and its right
+//   child appended (first in slot order on ties), until 8 slots or none is
+//   left; then repeatedly the child of largest area whose leaves fit in the
+//   slots left (a subtree of at most 8 leaves) is replaced by its leaves.
 // node(i) -> (box, left, right, count); returns the number of slots.
 template <class NodeF>
 WB_HD inline int wb_collapse(int root, NodeF node, int* ch) {
-    int n = 0;
     int l, r, cnt;
     WbBox box;
     node(root, box, l, r, cnt);
-    if (cnt > 0) { ch[n++] = root; }
-    else { ch[n++] = l; ch[n++] = r; }
+    if (cnt > 0) {
+        ch[0] = root;
+        return 1;
+    }
+    int n = wb_treelet(root, node, ch, WBVH_WIDTH);
+    if (n <= WBVH_WIDTH) return n;
+    int leaves[WBVH_WIDTH], tmp[WBVH_WIDTH];
+    n = 0;
+    ch[n] = l; leaves[n++] = wb_treelet(l, node, tmp, WBVH_WIDTH);
+    ch[n] = r; leaves[n++] = wb_treelet(r, node, tmp, WBVH_WIDTH);
     while (n < WBVH_WIDTH) {
         int best = -1;
         double ba = -1.0;
         for (int k = 0; k < n; k++) {
+            if (leaves[k] <= WBVH_WIDTH) continue;
             node(ch[k], box, l, r, cnt);
-            if (cnt > 0) continue;
             const double a = wb_area(box);
             if (a > ba) { ba = a; best = k; }
         }
         if (best < 0) break;
         node(ch[best], box, l, r, cnt);
-        ch[best] = l;
-        ch[n++] = r;
+        ch[best] = l; leaves[best] = wb_treelet(l, node, tmp, WBVH_WIDTH);
+        ch[n] = r; leaves[n++] = wb_treelet(r, node, tmp, WBVH_WIDTH);
+    }
+    for (;;) {
+        int best = -1;
+        double ba = -1.0;
+        for (int k = 0; k < n; k++) {
+            if (leaves[k] < 2 || n - 1 + leaves[k] > WBVH_WIDTH) continue;
+            node(ch[k], box, l, r, cnt);
+            const double a = wb_area(box);
+            if (a > ba) { ba = a; best = k; }
+        }
+        if (best < 0) break;
+        const int m = wb_treelet(ch[best], node, tmp, WBVH_WIDTH);
+        ch[best] = tmp[0]; leaves[best] = 1;
+        for (int k = 1; k < m; k++) { ch[n] = tmp[k]; leaves[n++] = 1; }
     }
     return n;
 }
