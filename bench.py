@@ -36,7 +36,7 @@ NODE_BYTES = 128   # one wide BVH node per fetch
 TRI_LEAF_BYTES = 48  # vertices + index of one triangle leaf slot
 # the device code the committed counter profiles were measured on
 KERNEL_SOURCES = ["render.hip", "kernels.h", "dev_trace.h", "dev_prims.h", "dev_shade.h", "dev_common.h",
-                  "sheen_math.h", "wbvh.h"]
+                  "sheen_math.h", "wbvh.h", "wbvh_rules.h", "wbvh_gpu.hip"]
 
 
 def kernel_sources_sha():
