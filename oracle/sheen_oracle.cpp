@@ -96,6 +96,8 @@ enum Plant {
     P_SIGMA = 31,          // strand extinction 5 instead of 6
     P_SHADOW_THRESH = 32,  // glass passes shadow rays from transmission 0.3
     P_EMISSION_LIT = 33,   // emission scaled by the ambient term
+    P_BEAM_NO_SHADOW = 34, // beam steps lit without their shadow rays (the reference's beam before
+                           // it traced them, ROADMAP.md:101-111)
 };
 #ifdef ORACLE_PLANTS
 static int g_plant = 0;
@@ -1838,7 +1840,7 @@ template <class R> inline V3<R> render_sample(const Scene& S, int x, int y, int 
             R remain = ldist * R(0.999f);
             Ray<R> sray{p, ldir, R(0.002f), remain};
             bool blocked = false;
-            for (int pass = 0; pass < 3; pass++) {
+            for (int pass = 0; pass < (PLANT(P_BEAM_NO_SHADOW) ? 0 : 3); pass++) {
                 Hit<R> srec;
                 cnt.shadow++;
                 if (!scene_intersect_opaque(S, sray, srec)) { blocked = false; break; }

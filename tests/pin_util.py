@@ -19,7 +19,7 @@ P = dict(POINT_DIST=1, NO_COS=2, TWO_PI=3, DISK_LINEAR=4, AREA_NO_COS=5, AREA_SI
          SPOT_HALF_ANGLE=9, SPEC_NL=10, SPEC_NO_GUARD=11, ROUGH_R2=12, IOR_RED=13, TRANS_NO_MASK=14, SKY_X=15,
          SKY_SWAP=16, REFL_NO_CLAMP=17, REFL_SWAP=18, ETA_FIXED=19, TIR_SIGN=20, W_ONCE=21, SHADOW_TMAX=22,
          SHADOW_4=23, CHORD_HALF=24, RADIUS_REV=25, TAU_NO_STOP=26, HF_HASH=27, HF_DETAIL=28, HF_LINEAR=29,
-         HF_OCT_SEED=30, SIGMA=31, SHADOW_THRESH=32, EMISSION_LIT=33)
+         HF_OCT_SEED=30, SIGMA=31, SHADOW_THRESH=32, EMISSION_LIT=33, BEAM_NO_SHADOW=34)
 
 THREADS = min(8, os.cpu_count() or 1)
 FOCAL, SENSOR = 1.0e7, 36.0  # near-orthographic: rays within 2e-6 rad of the axis
