@@ -377,6 +377,21 @@ def main():
     if PARAMS:
         line["params"] = PARAMS
     line["scene_setup"] = setup
+    # Not part of `value`: the same frames with the shadow requests whose
+    # unshadowed contribution is exactly zero left untraced (knob skip_dark;
+    # identical images, fewer rays than the reference traces)
+    if "skip_dark" not in PARAMS:
+        ctx.set_param("skip_dark", 1)
+        ctx.render(W, H, A, download=False)
+        rays = 0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, st = ctx.render(W, H, A, download=False)
+            rays += st["total_rays"]
+        dt = time.perf_counter() - t0
+        ctx.set_param("skip_dark", 0)
+        line["skip_dark"] = {"ms_per_frame": round(dt / args.steps * 1e3, 3), "rays_per_frame": rays // args.steps,
+                             "note": "same images; shadow rays of lights behind the shading normal not traced"}
     ctx.close()
     if not args.no_analytic:
         line["analytic"] = analytic_leg(max(2, min(args.steps, 5)))

@@ -249,7 +249,10 @@ int sheen_context_set_chunk(SheenContext* ctx, int samples_per_chunk);
  * also by the direction's dominant axis; a stable device sort, so a wave
  * traces rays of one kind heading one way; the image is the same either
  * way; default 2 for scenes of spheres, triangles and planes, 0 otherwise);
- * "shade_block"
+ * "skip_dark" (default 0; 1: a shadow request whose unshadowed contribution
+ * is exactly zero, a light behind the shading normal of a non-hair
+ * material, is not traced; the image is the same, the reference traces
+ * them and counts them as rays); "shade_block"
  * (workgroup size of the shading kernels, a multiple of 64 up to 1024; 0 =
  * the compiled bound, 1024 for scenes of spheres, triangles and planes, 512
  * otherwise).  Returns 0, or -1 for an unknown name or an invalid value. */

@@ -375,6 +375,13 @@ struct EventBufs {
 // changes no image.
 #define ORDER_SHADOW 1
 #define ORDER_SEC 2
+// SKIP_DARK: a shadow request whose unshadowed contribution is exactly zero
+// (a light behind the shading normal of a non-hair material: max(n.l, 0) = 0
+// and no specular without it) is not traced; the pixel adds 0 x
+// transmittance either way, so the image is the same, but fewer rays are
+// traced than the reference traces (off by default: the reference's ray
+// counts are the unit of work).
+#define ORDER_SKIP_DARK 4
 SHD uint32_t ev_of(const EventBufs& B, uint32_t slot, uint32_t kind) { return kind * B.cap + slot; }
 SHD uint32_t ev_light(const EventBufs& B, uint32_t ev, int l) { return ev + (uint32_t)l * 3u * B.cap; }
 // sample slot of a per-light entry (a shadow request's index)
@@ -487,7 +494,11 @@ SHD void shade_event(const DScene& S, bool active, const DHit& rec, v3 view_dir,
                 sd = normalize(target - so);
                 stmax = t_max;
                 if (debug) { debug_taken = true; debug_light = i; }
-                else contrib = shade_direct(S, mat, in, Ld, radiance, shininess);
+                else {
+                    contrib = shade_direct(S, mat, in, Ld, radiance, shininess);
+                    if ((B.order & ORDER_SKIP_DARK) && contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f)
+                        emit = false;
+                }
             }
         }
         if (has_light) {

@@ -1127,6 +1127,7 @@ int sheen_context_set_param(SheenContext* ctx, const char* name, int value) {
     }
     else if (n == "shadow_order") ctx->order = value ? (ctx->order | ORDER_SHADOW) : (ctx->order & ~ORDER_SHADOW);
     else if (n == "sec_order") ctx->order = value ? (ctx->order | ORDER_SEC) : (ctx->order & ~ORDER_SEC);
+    else if (n == "skip_dark") ctx->order = value ? (ctx->order | ORDER_SKIP_DARK) : (ctx->order & ~ORDER_SKIP_DARK);
     else if (n == "primary_grid") ctx->primary_grid = value;
     else if (n == "sec_grid") ctx->sec_grid = value;
     else if (n == "shadow_grid") ctx->shadow_grid = value;
@@ -1586,6 +1587,7 @@ int sheen_batch_set_param(SheenBatch* b, const char* name, int value) {
     else if (n == "graph") b->use_graph = value != 0;
     else if (n == "shadow_order") b->order = value ? (b->order | ORDER_SHADOW) : (b->order & ~ORDER_SHADOW);
     else if (n == "sec_order") b->order = value ? (b->order | ORDER_SEC) : (b->order & ~ORDER_SEC);
+    else if (n == "skip_dark") b->order = value ? (b->order | ORDER_SKIP_DARK) : (b->order & ~ORDER_SKIP_DARK);
     else if (n == "sec_sort") {
         if (value < 0 || value > 2) { sheen_set_error("sec_sort must be 0, 1 or 2"); return -1; }
         b->sec_sort = (int)value;

@@ -159,6 +159,25 @@ def test_sorted_secondary_rays_do_not_change_the_image(cfg):
             assert st[k] == st0[k], (key, k)
 
 
+@pytest.mark.parametrize("cfg", [1, 2, 100, 101, 102, 104, 105])
+def test_skipping_dark_shadow_rays_keeps_the_image(cfg):
+    """With `skip_dark`, shadow requests whose unshadowed contribution is
+    exactly zero are not traced: the image is bit-identical (they add
+    0 x transmittance) while fewer shadow rays are counted."""
+    from sheen_amd import scenes, sheen
+    sc = scenes.generate(cfg, **({"detail": 40} if cfg == 2 else {}))
+    w, h = (160, 120) if cfg != 1 else (128, 128)
+    with sheen.Context(sc.scene) as ctx:
+        ref, st0 = ctx.render(w, h, 4)
+        ctx.set_param("skip_dark", 1)
+        img, st1 = ctx.render(w, h, 4)
+        ctx.set_chunk(64 * 4 * 5)
+        img2, _ = ctx.render(w, h, 4)
+    assert np.array_equal(img, ref) and np.array_equal(img2, ref)
+    assert st1["shadow_rays"] <= st0["shadow_rays"]
+    assert st1["primary_rays"] == st0["primary_rays"] and st1["reflection_rays"] == st0["reflection_rays"]
+
+
 def test_autotuner_picks_from_its_space_and_keeps_the_image():
     from sheen_amd import scenes, sheen, autotune
     sc = scenes.generate(2, detail=40)
