@@ -751,8 +751,11 @@ __global__ TRACE_BOUNDS void k_shadow(SRC src, EventBufs EB, Counters* C, int pa
             if (res != 2) {
                 EB.trans[q.slot & 0x7FFFFFFFu] = (res == 1) ? 1.0f : 0.0f;
             } else if (k < EB.cont_cap) {
-                ShadowReq c = q;
-                c.ox = r.o.x; c.oy = r.o.y; c.oz = r.o.z;
+                // rebuilt from the ray (the same values: only the origin moved)
+                // so that only the slot stays live across the traversal
+                ShadowReq c;
+                c.ox = r.o.x; c.oy = r.o.y; c.oz = r.o.z; c.tmax = r.tmax;
+                c.dx = r.d.x; c.dy = r.d.y; c.dz = r.d.z; c.slot = q.slot;
                 EB.cont[0][k] = c;
             } else {
                 EB.trans[q.slot & 0x7FFFFFFFu] = shadow_rest<COUNT, false>(S, r, r, q.slot, 1, st, passes, ts);
@@ -800,8 +803,9 @@ __global__ TRACE_BOUNDS void k_shadow_cont(SRC src, EventBufs EB, Counters* C, i
         } else if (stage == 1) {
             EB.trans[slot] = 1.0f;  // three glass passes: unblocked
         } else if (k < EB.cont_cap) {
-            ShadowReq c = q;
-            c.ox = r.o.x; c.oy = r.o.y; c.oz = r.o.z;
+            ShadowReq c;  // as in k_shadow
+            c.ox = r.o.x; c.oy = r.o.y; c.oz = r.o.z; c.tmax = r.tmax;
+            c.dx = r.d.x; c.dy = r.d.y; c.dz = r.d.z; c.slot = q.slot;
             EB.cont[1][k] = c;
         } else {
             EB.trans[slot] = shadow_rest<COUNT, false>(S, r, r, q.slot, 2, st, passes, ts);
