@@ -782,9 +782,14 @@ static int render_frame(SheenContext* c, int W, int H, int aa, SheenRayStats* st
     if (fx) chunk = fx->chunk;
     else if (c->chunk_override > 0) chunk = (size_t)c->chunk_override;
     else {
-        const size_t budget = (size_t)8 << 30;  // 8 GiB of work buffers
+        // 32 GiB of work buffers (of 288 GB), at most 16 M slots: the analytic
+        // 10k frame (14.7 M slots) in one chunk measured 93.5 ms against
+        // 101 ms in two chunks of an 8 GiB budget
+        const size_t budget = (size_t)32 << 30;
         chunk = budget / slot_bytes(c->ds.num_lights, c->n_beam, caustic, c->ds.num_media > 0);
         chunk = std::min(chunk, (size_t)16 << 20);
+        // per-light event entries are indexed by 31 bits (ensure_bufs)
+        chunk = std::min(chunk, ((size_t)1 << 31) / (3 * (size_t)std::max(c->ds.num_lights, 1)) - 1);
     }
     chunk = std::max(unit, chunk / unit * unit);
     chunk = std::min(chunk, (total + unit - 1) / unit * unit);
