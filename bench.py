@@ -351,6 +351,8 @@ def main():
         "roofline": {
             "kernel": dom,
             "bound": "hbm",
+            # the unit the counters show limiting the kernel (counters.limiter)
+            "limiter": (counters or {}).get("limiter"),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
