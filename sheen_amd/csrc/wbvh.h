@@ -26,7 +26,13 @@
 
 #define WBVH_WIDTH 8
 #define WBVH_EMPTY 0xFFu
-#define WBVH_MAX_LEAF 4
+#ifndef WBVH_MAX_LEAF
+#define WBVH_MAX_LEAF 3
+#endif
+// a leaf slot's meta byte holds its count in 3 bits and its offset in the
+// node's primitive run in 5 bits; a node holds up to 8 leaves
+static_assert(WBVH_MAX_LEAF >= 1 && WBVH_MAX_LEAF <= 7 && WBVH_MAX_LEAF * WBVH_WIDTH <= 32,
+              "leaf offsets must fit the meta byte's 5 bits");
 #define WBVH_QMAX 2048  // largest child-plane integer (fp16-exact)
 
 struct __attribute__((aligned(128))) WNode {

@@ -625,7 +625,7 @@ bool wbvh_build_device(const float* box, int n, const unsigned char* skip, int m
     out = WBvhDevice();
     if (n <= 0) return true;
     if (max_leaf < 1) max_leaf = 1;
-    if (max_leaf > 7) max_leaf = 7;
+    if (max_leaf > 32 / WBVH_WIDTH) max_leaf = 32 / WBVH_WIDTH;  // leaf offsets fit 5 bits (wbvh.h)
     DevPool pool;
     const size_t cap_large = (size_t)n / SMALL_MAX + 2;
     const size_t cap_tasks = (size_t)n / CHUNK + cap_large + 2;
