@@ -348,7 +348,7 @@ bool host_prepare_scene(const SheenScene& s, HostScene& out, std::string& err, b
             }
         WBvhHost wb;
         try {
-            wbvh_build(mn.data(), mx.data(), counts[c], hb.skip.data(), WBVH_MAX_LEAF, 16, wb);
+            wbvh_build(mn.data(), mx.data(), counts[c], hb.skip.data(), WBVH_MAX_LEAF, WBVH_BINS, wb);
         } catch (const std::exception& e) {
             err = e.what();
             return false;

@@ -108,7 +108,7 @@ void wbvh_build(const float* bmin, const float* bmax, int n, const unsigned char
     out.leaf_count = 0;
     if (max_leaf < 1) max_leaf = 1;
     if (max_leaf > 32 / WBVH_WIDTH) max_leaf = 32 / WBVH_WIDTH;  // leaf offsets fit 5 bits (wbvh.h)
-    if (bins != WBVH_BINS) throw std::runtime_error("wbvh: the builder bins 16 ways");
+    if (bins != WBVH_BINS) throw std::runtime_error("wbvh: the builder bins WBVH_BINS ways");
     Builder B;
     B.max_leaf = max_leaf;
     B.pbox.resize(n);
