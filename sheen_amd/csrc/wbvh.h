@@ -34,6 +34,9 @@
 static_assert(WBVH_MAX_LEAF >= 1 && WBVH_MAX_LEAF <= 7 && WBVH_MAX_LEAF * WBVH_WIDTH <= 32,
               "leaf offsets must fit the meta byte's 5 bits");
 #define WBVH_QMAX 2048  // largest child-plane integer (fp16-exact)
+#ifndef WBVH_BINS
+#define WBVH_BINS 16  // binned-SAH bins per axis (32: 0.3 % faster frames, 30 % slower builds)
+#endif
 
 struct __attribute__((aligned(128))) WNode {
     float ox, oy, oz;

@@ -17,9 +17,6 @@
 #define WB_HD
 #endif
 
-#ifndef WBVH_BINS
-#define WBVH_BINS 16
-#endif
 
 struct WbBox {
     float mn[3], mx[3];
