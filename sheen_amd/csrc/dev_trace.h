@@ -323,10 +323,14 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
         if (LEAF_BATCH && share) {
             const uint32_t lane = __lane_id();
             const uint64_t lt = (1ull << lane) - 1ull;
-            // tests of the leaf slots in `take`, and their offset among the wave's
-            auto count = [&](uint32_t take, uint32_t& off, uint32_t& N) -> uint32_t {
+            // tests of the leaf slots in `take`
+            auto tests = [&](uint32_t take) -> uint32_t {
                 uint32_t n = 0;
                 for (uint32_t lm = take; lm; lm &= lm - 1) n += ((uint32_t)(meta >> (8 * __builtin_ctz(lm))) & 0xFFu) >> 5;
+                return n;
+            };
+            // offset of this lane's n tests among the wave's, and the wave's total
+            auto prefix = [&](uint32_t n, uint32_t& off, uint32_t& N) {
                 off = 0;
                 N = 0;
 #pragma unroll
@@ -336,19 +340,22 @@ SHD bool trav_step(const DScene& S, int cat, bool any, const DRay& r, v3 inv, Tr
                     off += __builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u)) << b;
                     N += (uint32_t)__popcll(mb) << b;
                 }
-                return n;
             };
             // Rounds: every pending leaf at once when the wave's tests fit the
             // list, else each lane's lowest pending leaf (a bottom node holds
-            // up to 8 leaves); single tests are left to the lanes themselves.
+            // up to 8 leaves); single tests are left to the lanes themselves
+            // (the common case of a visit with no leaf, or one test per lane,
+            // costs one ballot).
             for (;;) {
                 const uint64_t active = __ballot(1);
                 uint32_t take = leafmask, off, N;
-                uint32_t n = count(take, off, N);
+                uint32_t n = tests(take);
                 if (__ballot(n >= 2) == 0ull) break;
+                prefix(n, off, N);
                 if (N > LEAF_LIST) {
                     take = leafmask & (0u - leafmask);
-                    n = count(take, off, N);
+                    n = tests(take);
+                    prefix(n, off, N);
                     if (N > LEAF_LIST) break;
                 }
                 for (uint32_t lm = take, j = off; lm; lm &= lm - 1) {
