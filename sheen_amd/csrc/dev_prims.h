@@ -146,10 +146,8 @@ SHD bool hit_triangle(const DRay& r, const Triangle& tri, DHit& h) {
 
 // Cone frustum segment (reference primitives/cone.h:11-73): cone_solve holds
 // the roots of a frustum once its frame is set up (the rest of hit_cone).
-SHD bool cone_solve(const DRay& r, v3 Dp, v3 Qp, v3 da, float u, float v, float k, float r0, float L, DHit& h) {
-    float A = dot(Dp, Dp) - k * k * u * u;
-    float B = 2.0f * (dot(Dp, Qp) - k * r0 * u - k * k * v * u);
-    float C = dot(Qp, Qp) - k * k * v * v - 2.0f * k * r0 * v - r0 * r0;
+SHD bool cone_roots(const DRay& r, float A, float B, float C, v3 Dp, v3 Qp, v3 da, float u, float v, float L,
+                    DHit& h) {
     float disc = B * B - 4.0f * A * C;
     if (disc < 0.0f) return false;
     float sq = sqrtf(disc);
@@ -169,6 +167,12 @@ SHD bool cone_solve(const DRay& r, v3 Dp, v3 Qp, v3 da, float u, float v, float 
         return true;
     }
     return false;
+}
+SHD bool cone_solve(const DRay& r, v3 Dp, v3 Qp, v3 da, float u, float v, float k, float r0, float L, DHit& h) {
+    float A = dot(Dp, Dp) - k * k * u * u;
+    float B = 2.0f * (dot(Dp, Qp) - k * r0 * u - k * k * v * u);
+    float C = dot(Qp, Qp) - k * k * v * v - 2.0f * k * r0 * v - r0 * r0;
+    return cone_roots(r, A, B, C, Dp, Qp, da, u, v, L, h);
 }
 // `smax`: a caller that only takes hits at distances up to smax (inclusive)
 // passes it to skip a frustum whose axial coordinate tp = v + s*u, for s in
@@ -199,7 +203,7 @@ SHD bool hit_cone(const DRay& r, v3 p0, float r0, v3 p1, float r1, DHit& h, floa
 // then 0 / L), so the products with them are left out: every value below
 // equals hit_cone's up to the sign of a zero, which changes no comparison and
 // no nonzero result, so the roots and the accepted hit are the same.
-SHD bool hit_cone_y(const DRay& r, float y0, float r0, float y1, float r1, DHit& h, float smax) {
+SHD bool hit_cone_y_div(const DRay& r, float y0, float r0, float y1, float r1, DHit& h, float smax) {
     const float ay = y1 - y0;
     const float L2 = ay * ay;
     if (L2 < 1e-12f) return false;
@@ -215,6 +219,45 @@ SHD bool hit_cone_y(const DRay& r, float y0, float r0, float y1, float r1, DHit&
     const v3 Dp = mk3(r.d.x, r.d.y - u * day, r.d.z);
     const v3 Qp = mk3(r.o.x, Qy - v * day, r.o.z);
     return cone_solve(r, Dp, Qp, mk3(0.0f, day, 0.0f), u, v, k, r0, L, h);
+}
+// The ray's own terms of every y-axis frustum (ConeYRay, set once per lathe
+// test): Dp = (d.x, 0, d.z) and Qp = (o.x, 0, o.z) whatever the frustum,
+// since day = +-1 (below) makes d.y - u*day and Qy - v*day exact zeros (+0:
+// x - x), so dot(Dp, Dp), dot(Dp, Qp) and dot(Qp, Qp) are the same sums of
+// the same products for all of them.
+struct ConeYRay {
+    float DD, DQ, QQ;
+};
+SHD ConeYRay cone_y_ray(const DRay& r) {
+    const v3 Dp = mk3(r.d.x, 0.0f, r.d.z), Qp = mk3(r.o.x, 0.0f, r.o.z);
+    return {dot(Dp, Dp), dot(Dp, Qp), dot(Qp, Qp)};
+}
+// hit_cone_y_div with the square root and divisions of its set-up made
+// exact: in binary floating point the square root of a rounded square is the
+// magnitude (sqrtf(ay*ay) == |ay| whenever ay*ay is a normal float; checked
+// over every float of [1, 4), which covers every exponent by scaling), so
+// L = |ay| and day = ay / |ay| = +-1 exactly.  A square that overflows takes
+// hit_cone_y_div.  Same roots, same accepted hit.
+SHD bool hit_cone_y(const DRay& r, const ConeYRay& cr, float y0, float r0, float y1, float r1, DHit& h, float smax) {
+    const float ay = y1 - y0;
+    const float L2 = ay * ay;
+    if (L2 < 1e-12f) return false;
+    if (!(L2 <= 0x1.fffffep127f)) return hit_cone_y_div(r, y0, r0, y1, r1, h, smax);
+    const float L = fabsf(ay);
+    const float day = ay < 0.0f ? -1.0f : 1.0f;
+    const float Qy = r.o.y - y0;
+    const float u = r.d.y * day, v = Qy * day;
+    {
+        const float ta = v + r.tmin * u, tb = v + smax * u;
+        if ((ta < 0.0f && tb < 0.0f) || (ta > L && tb > L)) return false;
+    }
+    const float k = (r1 - r0) / L;
+    const v3 Dp = mk3(r.d.x, r.d.y - u * day, r.d.z);
+    const v3 Qp = mk3(r.o.x, Qy - v * day, r.o.z);
+    const float A = cr.DD - k * k * u * u;
+    const float B = 2.0f * (cr.DQ - k * r0 * u - k * k * v * u);
+    const float C = cr.QQ - k * k * v * v - 2.0f * k * r0 * v - r0 * r0;
+    return cone_roots(r, A, B, C, Dp, Qp, mk3(0.0f, day, 0.0f), u, v, L, h);
 }
 // Half sphere on the side of `axis` (reference primitives/cone.h:78-121);
 // leaves h.tangent untouched, as the reference does.
@@ -566,11 +609,12 @@ SHD bool hit_lathe(const DRay& r, const Lathe& l, const LathePoint* pts, DHit& h
     bool any = false;
     float closest = r.tmax;
     DHit tmp;
+    const ConeYRay cr = cone_y_ray(r);
     for (int i = 0; i < l.point_count - 1; i++) {
         const LathePoint& p0 = pts[l.point_offset + i];
         const LathePoint& p1 = pts[l.point_offset + i + 1];
         if (p0.radius <= 0.0f && p1.radius <= 0.0f) continue;
-        if (hit_cone_y(r, p0.y, p0.radius, p1.y, p1.radius, tmp, fminf(smax, closest)) && tmp.t < closest) {
+        if (hit_cone_y(r, cr, p0.y, p0.radius, p1.y, p1.radius, tmp, fminf(smax, closest)) && tmp.t < closest) {
             tmp.point = r.o + tmp.t * r.d;
             h = tmp;
             h.material_id = l.material_id;
