@@ -77,6 +77,23 @@ struct CsgSpineNode {
     int b;
 };
 
+// A load through the scalar cache: the address must be the same in every
+// active lane (a value the wave reads once), and the memory read-only while
+// the kernel runs.
+template <class T>
+SHD T sload(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "whole dwords");
+    typedef const __attribute__((address_space(4))) uint32_t* CW;
+    const CW q = (CW)p;
+    T v;
+    uint32_t* w = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); i++) w[i] = q[i];
+    return v;
+}
+
+struct LatheFrustum;  // dev_prims.h
+
 // Device scene: everything a kernel reads, passed by value.
 struct DScene {
     const WNode* bvh[N_BVH_CATS];
@@ -96,6 +113,11 @@ struct DScene {
     const Superellipsoid* ses;
     const Lathe* lathes;
     const LathePoint* lathe_points;
+    const LatheFrustum* lathe_frusta;  // per profile point: the frustum to the next point (lathe_frustum)
+    // lathes of the BVH in the order lathe_scan tests them, widest profile
+    // first (lathe_scan of them; 0: the tree is traversed)
+    const int* lathe_order;
+    int lathe_scan;
     const Prism* prisms;
     const PrismPoint* prism_points;
     const CSGNode* csg;

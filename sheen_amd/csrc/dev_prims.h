@@ -259,6 +259,50 @@ SHD bool hit_cone_y(const DRay& r, const ConeYRay& cr, float y0, float r0, float
     const float C = cr.QQ - k * k * v * v - 2.0f * k * r0 * v - r0 * r0;
     return cone_roots(r, A, B, C, Dp, Qp, mk3(0.0f, day, 0.0f), u, v, L, h);
 }
+// Per-frustum terms of a lathe profile (LatheFrustum, built once per scene
+// by lathe_frustum): hit_cone_y's set-up, and the products of k and r0 that
+// cone_solve forms the same way for every ray: k*k, k*r0, (2*k)*r0, r0*r0.
+// Mode 1: the frustum is never hit (both radii <= 0, or L2 < 1e-12); mode 2:
+// the square ay*ay overflows, and the frustum takes hit_cone_y_div.
+struct LatheFrustum {
+    float y0, L, day, k2;
+    float kr0, tkr0, r02;
+    int mode;
+};
+static_assert(sizeof(LatheFrustum) == 32, "one 32-byte record per frustum");
+SHHD LatheFrustum lathe_frustum(float y0, float r0, float y1, float r1) {
+    LatheFrustum f{};
+    f.y0 = y0;
+    const float ay = y1 - y0;
+    const float L2 = ay * ay;
+    if ((r0 <= 0.0f && r1 <= 0.0f) || L2 < 1e-12f) { f.mode = 1; return f; }
+    if (!(L2 <= 0x1.fffffep127f)) { f.mode = 2; return f; }
+    f.L = fabsf(ay);
+    f.day = ay < 0.0f ? -1.0f : 1.0f;
+    const float k = (r1 - r0) / f.L;
+    f.k2 = k * k;
+    f.kr0 = k * r0;
+    f.tkr0 = 2.0f * k * r0;
+    f.r02 = r0 * r0;
+    return f;
+}
+// hit_cone_y from a frustum record of mode 0: the same operations on the
+// same operands, so the same roots and the same accepted hit.
+SHD bool hit_cone_y_rec(const DRay& r, const ConeYRay& cr, const LatheFrustum& f, DHit& h, float smax) {
+    const float L = f.L, day = f.day;
+    const float Qy = r.o.y - f.y0;
+    const float u = r.d.y * day, v = Qy * day;
+    {
+        const float ta = v + r.tmin * u, tb = v + smax * u;
+        if ((ta < 0.0f && tb < 0.0f) || (ta > L && tb > L)) return false;
+    }
+    const v3 Dp = mk3(r.d.x, r.d.y - u * day, r.d.z);
+    const v3 Qp = mk3(r.o.x, Qy - v * day, r.o.z);
+    const float A = cr.DD - f.k2 * u * u;
+    const float B = 2.0f * (cr.DQ - f.kr0 * u - f.k2 * v * u);
+    const float C = cr.QQ - f.k2 * v * v - f.tkr0 * v - f.r02;
+    return cone_roots(r, A, B, C, Dp, Qp, mk3(0.0f, day, 0.0f), u, v, L, h);
+}
 // Half sphere on the side of `axis` (reference primitives/cone.h:78-121);
 // leaves h.tangent untouched, as the reference does.
 SHD bool hit_cap(const DRay& r, v3 c, float radius, v3 axis, DHit& h) {

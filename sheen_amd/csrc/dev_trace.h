@@ -680,6 +680,74 @@ struct QHit {
     int carry;  // object_id carried into a CSG hit
 };
 
+// Lathes of a scene with at most LATHE_SCAN_MAX of them are tested without
+// their tree, in one fixed order (`lathe_order`, widest profile first), so
+// the lathe and frustum in hand are the same in every lane of a wave and are
+// read through the scalar cache; the frusta come from the scene's records
+// (lathe_frustum).  The result is the traversal's: both take the
+// lexicographic minimum of (distance, index) over the lathes whose reference
+// bounds the ray passes (leaf_dist's gate), with the running best as the
+// frusta's bound (hit_lathe's `smax`), and the tree culls little for
+// lathes, whose reference bounds span +-1e30 in x and z.  With ANY (no
+// lathe material passes shadow rays) a lane stops at its first hit nearer
+// than its bound, which the traversal would take as well.
+#ifndef LATHE_SCAN_MAX
+#define LATHE_SCAN_MAX 64
+#endif
+template <bool ANY>
+SHD bool lathe_frusta_hit(const DScene& S, const DRay& r, const ConeYRay& cr, const Lathe& l, float smax, float& t) {
+    bool any = false;
+    float closest = r.tmax;
+    DHit tmp;
+    for (int i = 0; i < l.point_count - 1; i++) {
+        const int g = __builtin_amdgcn_readfirstlane(l.point_offset + i);
+        const LatheFrustum f = sload(S.lathe_frusta + g);
+        if (f.mode == 1) continue;
+        bool hit;
+        if (f.mode == 0) {
+            hit = hit_cone_y_rec(r, cr, f, tmp, fminf(smax, closest));
+        } else {
+            const LathePoint p0 = sload(S.lathe_points + g), p1 = sload(S.lathe_points + g + 1);
+            hit = hit_cone_y_div(r, p0.y, p0.radius, p1.y, p1.radius, tmp, fminf(smax, closest));
+        }
+        if (hit && tmp.t < closest) {
+            closest = tmp.t;
+            any = true;
+            if (ANY && closest < smax) break;
+        }
+    }
+    t = closest;
+    return any;
+}
+template <bool ANY, bool COUNT>
+SHD void lathe_scan(const DScene& S, const DRay& r, v3 inv, float& ct, int& ci, bool& ch, TraceStats& ts) {
+    const ConeYRay cr = cone_y_ray(r);
+    for (int j = 0; j < S.lathe_scan; j++) {
+        const int idx = __builtin_amdgcn_readfirstlane(sload(S.lathe_order + __builtin_amdgcn_readfirstlane(j)));
+        const float* bb = S.bounds[C_LATHE] + 6 * (size_t)idx;
+        const float mn[3] = {sload(bb), sload(bb + 1), sload(bb + 2)};
+        const float mx[3] = {sload(bb + 3), sload(bb + 4), sload(bb + 5)};
+        if (!ref_slab(r, inv, mn, mx)) continue;
+        const Lathe l = sload(S.lathes + idx);
+        if (l.point_count < 2) continue;
+        if (COUNT) ts.prims++;
+        float t;
+        if (!lathe_frusta_hit<ANY>(S, r, cr, l, ct, t)) continue;
+        if (t < ct || (t == ct && ch && idx < ci)) {
+            ct = t;
+            ci = idx;
+            ch = true;
+            if (ANY) return;
+        }
+    }
+}
+template <int CAT, bool ANY, bool COUNT>
+SHD void cat_traverse(const DScene& S, const DRay& r, v3 inv, float& ct, int& ci, bool& ch, TStack& st,
+                      TraceStats& ts) {
+    if (CAT == C_LATHE && S.lathe_scan > 0) lathe_scan<ANY, COUNT>(S, r, inv, ct, ci, ch, ts);
+    else traverse<CAT, ANY, COUNT>(S, r, inv, ct, ci, ch, st, ts);
+}
+
 template <int CAT, bool COUNT>
 SHD void closest_cat(const DScene& S, const DRay& r, v3 inv, float init_t, QHit& q, float& best, int& last_obj,
                      TStack& st, TraceStats& ts) {
@@ -687,7 +755,7 @@ SHD void closest_cat(const DScene& S, const DRay& r, v3 inv, float init_t, QHit&
     float ct = init_t;
     int ci = -1;
     bool ch = false;
-    traverse<CAT, false, COUNT>(S, r, inv, ct, ci, ch, st, ts);
+    cat_traverse<CAT, false, COUNT>(S, r, inv, ct, ci, ch, st, ts);
     if (ch) {
         last_obj = (CAT == C_SPHERE) ? ci : -(ci + 1);
         if (ct < best) { best = ct; q.t = ct; q.kind = CAT; q.idx = ci; }
@@ -748,8 +816,8 @@ SHD bool opaque_cat(const DScene& S, const DRay& r, v3 inv, QHit& q, TStack& st,
     float ct = r.tmax;
     int ci = -1;
     bool ch = false;
-    if (S.cat_opaque[CAT]) traverse<CAT, true, COUNT>(S, r, inv, ct, ci, ch, st, ts);
-    else traverse<CAT, false, COUNT>(S, r, inv, ct, ci, ch, st, ts);
+    if (S.cat_opaque[CAT]) cat_traverse<CAT, true, COUNT>(S, r, inv, ct, ci, ch, st, ts);
+    else cat_traverse<CAT, false, COUNT>(S, r, inv, ct, ci, ch, st, ts);
     if (ch) { q.t = ct; q.kind = CAT; q.idx = ci; }
     return ch;
 }

@@ -280,6 +280,18 @@ static_assert(HOST_CSG_DEPTH == CSG_MAX_DEPTH, "host and device CSG stack depths
 static_assert(sizeof(HostSpineNode) == sizeof(CsgSpineNode), "spine node layout");
 
 // Triangle leaf copies in leaf-slot order: v0 + index, v1, v2 (48 bytes).
+// Frustum records of the lathe profiles (lathe_frustum), one thread per lathe;
+// point ranges were validated on the host, and are clipped here as well.
+__global__ void k_lathe_frusta(const Lathe* lathes, int n, const LathePoint* pts, int n_pts, LatheFrustum* out) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const Lathe L = lathes[l];
+    for (int i = 0; i + 1 < L.point_count; i++) {
+        const int g = L.point_offset + i;
+        if (g < 0 || g + 1 >= n_pts) break;
+        out[g] = lathe_frustum(pts[g].y, pts[g].radius, pts[g + 1].y, pts[g + 1].radius);
+    }
+}
 __global__ void k_tri_leaf(const uint32_t* prims, int n, const Triangle* tris, float4* out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -456,6 +468,33 @@ static SheenContext* context_create(const SheenScene* in, int flags = 0) {
     d.ses = c->upload(in->superellipsoids, in->superellipsoids ? (size_t)in->num_superellipsoids : 0);
     d.lathes = c->upload(in->lathes, in->lathes ? (size_t)in->num_lathes : 0);
     d.lathe_points = c->upload(in->lathe_points, in->lathe_points ? (size_t)in->num_lathe_points : 0);
+    // up to LATHE_SCAN_MAX lathes are tested without their tree, widest
+    // profile first, from per-frustum records (lathe_scan in dev_trace.h)
+    if (d.count[C_LATHE] > 0 && d.lathe_points && c->bvh_prims[C_LATHE] > 0 && c->bvh_prims[C_LATHE] <= LATHE_SCAN_MAX) {
+        const int n = c->bvh_prims[C_LATHE];
+        std::vector<uint32_t> ids((size_t)n);
+        HIP_OR(hipStreamSynchronize(c->stream), nullptr);
+        HIP_OR(hipMemcpy(ids.data(), d.prim_ids[C_LATHE], (size_t)n * 4, hipMemcpyDeviceToHost), nullptr);
+        std::vector<float> width((size_t)n, 0.0f);
+        for (int j = 0; j < n; j++) {
+            if (ids[j] >= (uint32_t)in->num_lathes) { sheen_set_error("lathe BVH index out of range"); return nullptr; }
+            const Lathe& l = in->lathes[ids[j]];
+            for (int k = 0; k < l.point_count; k++) width[j] = std::max(width[j], in->lathe_points[l.point_offset + k].radius);
+        }
+        std::vector<int> order((size_t)n);
+        for (int j = 0; j < n; j++) order[j] = j;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return width[a] > width[b]; });
+        std::vector<int> sorted((size_t)n);
+        for (int j = 0; j < n; j++) sorted[j] = (int)ids[order[j]];
+        LatheFrustum* fr = c->alloc_owned<LatheFrustum>((size_t)in->num_lathe_points);
+        d.lathe_order = c->upload(sorted.data(), sorted.size());
+        if (!fr || !d.lathe_order) { sheen_set_error("lathe record upload failed"); return nullptr; }
+        k_lathe_frusta<<<(in->num_lathes + 255) / 256, 256, 0, c->stream>>>(d.lathes, in->num_lathes, d.lathe_points,
+                                                                            in->num_lathe_points, fr);
+        HIP_OR(hipGetLastError(), nullptr);
+        d.lathe_frusta = fr;
+        d.lathe_scan = n;
+    }
     d.prisms = c->upload(in->prisms, in->prisms ? (size_t)in->num_prisms : 0);
     d.prism_points = c->upload(in->prism_points, in->prism_points ? (size_t)in->num_prism_points : 0);
     d.csg = c->upload(in->csg_nodes, in->csg_nodes ? (size_t)in->num_csg_nodes : 0);
