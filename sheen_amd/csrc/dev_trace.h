@@ -34,6 +34,15 @@
 #else
 #define TRACE_BOUNDS __launch_bounds__(TRACE_BLOCK)
 #endif
+// the instantiations for every primitive kind (FULL) may take another target
+#ifndef TRACE_MIN_WAVES_FULL
+#define TRACE_MIN_WAVES_FULL TRACE_MIN_WAVES
+#endif
+#if TRACE_MIN_WAVES > 0 && TRACE_MIN_WAVES_FULL > 0
+#define TRACE_BOUNDS_OF(FULL) __launch_bounds__(TRACE_BLOCK, (FULL) ? TRACE_MIN_WAVES_FULL : TRACE_MIN_WAVES)
+#else
+#define TRACE_BOUNDS_OF(FULL) TRACE_BOUNDS
+#endif
 // The shading kernels run 1024-thread blocks: their queue reservations take
 // one device atomic per block (block_reserve), and each light's shadow
 // requests of a block land in one contiguous run of the queue.  On the

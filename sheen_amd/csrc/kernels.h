@@ -461,7 +461,7 @@ SHD int4 pack_hit(const QHit& q) {
 }
 
 template <bool COUNT, bool FULL, class SRC>
-__global__ TRACE_BOUNDS void k_primary(SRC src, int4* hit_p, Counters* C) {
+__global__ TRACE_BOUNDS_OF(FULL) void k_primary(SRC src, int4* hit_p, Counters* C) {
     SCENE_TABLE(src);
     SHEEN_STACK(st);
     TraceStats ts{0, 0};
@@ -584,7 +584,7 @@ __global__ SHADE_BOUNDS(FULL) void k_shade_primary(SRC src, SampleBufs SB, Event
 }
 
 template <bool COUNT, bool FULL, class SRC>
-__global__ TRACE_BOUNDS void k_trace_sec(SRC src, const SecRay* q_in, int which, int4* hits, Counters* C) {
+__global__ TRACE_BOUNDS_OF(FULL) void k_trace_sec(SRC src, const SecRay* q_in, int which, int4* hits, Counters* C) {
     SCENE_TABLE(src);
     SHEEN_STACK(st);
     TraceStats ts{0, 0};
@@ -703,7 +703,7 @@ __global__ SHADE_BOUNDS(FULL) void k_shade_sec(SRC src, SampleBufs SB, EventBufs
 }
 
 template <bool COUNT, bool FULL, class SRC>
-__global__ TRACE_BOUNDS void k_shadow(SRC src, EventBufs EB, Counters* C, int part) {
+__global__ TRACE_BOUNDS_OF(FULL) void k_shadow(SRC src, EventBufs EB, Counters* C, int part) {
     SCENE_TABLE(src);
     SHEEN_STACK(st);
     TraceStats ts{0, 0};
