@@ -594,7 +594,7 @@ SHD void csg_eval(const DScene& S, const DRay& r, v3 inv, float t0, float t1, in
 // spans that end beyond t0 — a point span at t0 counts when E = t0.  The
 // subtrees are therefore visited near-first through the spine hierarchy and
 // one whose box entry lies beyond the best E found so far is skipped.
-SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& best_iv, float& t_out) {
+SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& best_iv, float& t_out, TStack& st) {
     const v3 inv = mk3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
     CSpan ret[8];
     int nret = 0;
@@ -631,8 +631,10 @@ SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& be
     };
     for (int k = 0; k < S.spine_always; k++) take(S.spine_sub[k]);
     if (S.spine_n_nodes > 0) {
-        int stk[32];
-        float stn[32];
+        // the spine's stack of {node, entry distance bits} in the traversal
+        // stack (LDS first; the category traversals are over), its spill
+        // count left as the traversals had it
+        const uint32_t spills0 = st.spills;
         int sp = 0;
         const CsgSpineNode* N = S.spine_nodes;
         auto node_box = [&](int i, float& tn, float& tf) {
@@ -641,13 +643,13 @@ SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& be
             return csg_line_box(r, inv, bx, tn, tf) && !(tn > t1 || tf < t0);
         };
         float tn0, tf0;
-        if (node_box(0, tn0, tf0)) { stk[0] = 0; stn[0] = tn0; sp = 1; }
+        if (node_box(0, tn0, tf0)) stack_push(st, sp, make_uint2(0u, __float_as_uint(tn0)));
         // the hierarchy is balanced (median splits): at most log2(subtrees) + 1
         // entries are ever stacked, far below 32
         while (sp > 0) {
-            sp--;
-            const int i = stk[sp];
-            if (stn[sp] > E) continue;
+            const uint2 se = stack_pop(st, sp);
+            const int i = (int)se.x;
+            if (__uint_as_float(se.y) > E) continue;
             const CsgSpineNode nd = N[i];
             if (nd.a < 0) {
                 for (int k = 0; k < nd.b; k++) take(S.spine_sub[-nd.a - 1 + k]);
@@ -666,11 +668,10 @@ SHD bool csg_query(const DScene& S, const DRay& r, float t0, float t1, CSpan& be
             for (int q = 1; q >= 0; q--) {
                 const int c = q == 0 ? first : 1 - first;
                 if (!hit[c] || sp >= 32) continue;
-                stk[sp] = ch[c];
-                stn[sp] = tnc[c];
-                sp++;
+                stack_push(st, sp, make_uint2((uint32_t)ch[c], __float_as_uint(tnc[c])));
             }
         }
+        st.spills = spills0;
     }
     if (!(E < 1e30f)) return false;
     CSpan head = e_iv;
@@ -808,7 +809,7 @@ SHD void query_closest(const DScene& S, const DRay& r, QHit& q, TStack& st, Trac
     if (FULL && S.csg_root >= 0) {
         CSpan iv;
         float t;
-        if (csg_query(S, r, r.tmin, r.tmax, iv, t) && t < best) {
+        if (csg_query(S, r, r.tmin, r.tmax, iv, t, st) && t < best) {
             best = t; q.t = t; q.kind = C_CSG; q.idx = csg_hit_leaf(iv); q.carry = last_obj;
         }
     }
@@ -849,7 +850,7 @@ SHD bool query_opaque(const DScene& S, const DRay& r, QHit& q, TStack& st, Trace
     if (FULL && S.csg_root >= 0) {
         CSpan iv;
         float t;
-        if (csg_query(S, r, r.tmin, r.tmax, iv, t)) { q.t = t; q.kind = C_CSG; q.idx = csg_hit_leaf(iv); q.carry = 0; return true; }
+        if (csg_query(S, r, r.tmin, r.tmax, iv, t, st)) { q.t = t; q.kind = C_CSG; q.idx = csg_hit_leaf(iv); q.carry = 0; return true; }
     }
     bool any = false;
     float closest = r.tmax;
