@@ -559,6 +559,13 @@ SHD void csg_eval(const DScene& S, const DRay& r, v3 inv, float t0, float t1, in
             continue;
         }
         if (f.state == 1) {  // left result in ret; evaluate right
+            if ((n.op == 1 || n.op == 2) && nret == 0) {
+                // an intersection or difference with an empty left operand is
+                // empty (csg_combine returns no span whatever the right one
+                // holds), so the right subtree is not evaluated
+                top--;
+                continue;
+            }
             f.nA = nret;
             for (int i = 0; i < nret; i++) f.A[i] = ret[i];
             f.state = 2;
