@@ -18,6 +18,10 @@ from util import PyScene, f3, material  # noqa: E402
 
 from sheen_amd import contract as K  # noqa: E402
 
+# lathes beyond this count are traversed through their tree; up to it they
+# are scanned in one fixed order (dev_trace.h LATHE_SCAN_MAX)
+LATHE_SCAN_MAX = 64
+
 pytestmark = pytest.mark.gpu
 
 CAMERAS = [  # position, forward, up
@@ -27,7 +31,7 @@ CAMERAS = [  # position, forward, up
 ]
 
 
-def _scene(cam):
+def _scene(cam, n_lathes=8):
     rng = np.random.default_rng(5)
     ps = PyScene()
     ps.set("materials", K.Material, [material(color=(0.8, 0.7, 0.6), spec=0.0), material(color=(0.3, 0.6, 0.9))],
@@ -45,13 +49,13 @@ def _scene(cam):
     p.normal, p.d, p.material_id = f3(0, 1, 0), 2.0, 0
     ps.set("planes", K.Plane, [p], "num_planes")
     pts, lathes = [], []
-    for k in range(8):
+    for k in range(n_lathes):
         la = K.Lathe()
         la.center = f3(*(rng.uniform(-2, 2, 3) if k % 3 else (0.0, 0.0, 0.0)))
         la.point_offset, la.material_id = len(pts), k % 2
         n = int(rng.integers(6, 34))
         y = np.sort(rng.uniform(-2.0, 4.0, n))
-        rad = np.abs(np.cumsum(rng.normal(0, 0.3, n))) + 0.2 * k
+        rad = np.abs(np.cumsum(rng.normal(0, 0.3, n))) + 0.2 * (k % 8)
         if k == 3:
             rad[n // 2] = -0.4  # a negative profile radius
         if k == 5:
@@ -67,11 +71,12 @@ def _scene(cam):
     return ps
 
 
-@pytest.mark.parametrize("cam", range(len(CAMERAS)))
-def test_lathe_hits_match_oracle(cam):
+@pytest.mark.parametrize("cam,n_lathes", [(c, 8) for c in range(len(CAMERAS))] + [(0, LATHE_SCAN_MAX), (2, LATHE_SCAN_MAX + 8)])
+def test_lathe_hits_match_oracle(cam, n_lathes):
+    """Scanned (up to LATHE_SCAN_MAX lathes) and tree-traversed lathes."""
     from oracle import Oracle
     from sheen_amd import sheen
-    ps = _scene(CAMERAS[cam])
+    ps = _scene(CAMERAS[cam], n_lathes)
     W, H, AA = 160, 120, 2
     with sheen.Context(ps.ptr) as ctx:
         img, _ = ctx.render(W, H, AA)
