@@ -123,6 +123,9 @@ struct DScene {
     const CSGNode* csg;
     const float* csg_box;  // 6 floats per CSG node (min xyz, max xyz): contains every span of its subtree
     int csg_root;
+    // 1: lathes, prisms and CSG leaves all have materials that block shadow
+    // rays (query_opaque may then ask them in any order)
+    int opaque_tail;
     int num_csg;
     // union spine of the CSG root (csg_spine != 0): subtrees as {node, in-order
     // rank, frame depth, 0}; the first spine_always of them have unbounded

@@ -848,16 +848,29 @@ SHD bool query_opaque(const DScene& S, const DRay& r, QHit& q, TStack& st, Trace
     q.kind = -1; q.idx = -1; q.carry = 0; q.t = 0.0f;
     if (opaque_cat<C_SPHERE, COUNT>(S, r, inv, q, st, ts)) return true;
     if (opaque_cat<C_TRI, COUNT>(S, r, inv, q, st, ts)) return true;
+    auto csg = [&]() {
+        CSpan iv;
+        float t;
+        if (!csg_query(S, r, r.tmin, r.tmax, iv, t, st)) return false;
+        q.t = t; q.kind = C_CSG; q.idx = csg_hit_leaf(iv); q.carry = 0;
+        return true;
+    };
     if (FULL) {
         if (opaque_cat<C_HF, COUNT>(S, r, inv, q, st, ts)) return true;
         if (opaque_cat<C_SE, COUNT>(S, r, inv, q, st, ts)) return true;
-        if (opaque_cat<C_LATHE, COUNT>(S, r, inv, q, st, ts)) return true;
-        if (opaque_cat<C_PRISM, COUNT>(S, r, inv, q, st, ts)) return true;
-    }
-    if (FULL && S.csg_root >= 0) {
-        CSpan iv;
-        float t;
-        if (csg_query(S, r, r.tmin, r.tmax, iv, t, st)) { q.t = t; q.kind = C_CSG; q.idx = csg_hit_leaf(iv); q.carry = 0; return true; }
+        if (S.opaque_tail) {
+            // lathes, prisms and the CSG tree all block every shadow ray they
+            // hit, so whichever of them reports a hit first, the pass is
+            // blocked as when the first of them in the reference's order
+            // reports it: the cheaper ones are asked first
+            if (opaque_cat<C_PRISM, COUNT>(S, r, inv, q, st, ts)) return true;
+            if (S.csg_root >= 0 && csg()) return true;
+            if (opaque_cat<C_LATHE, COUNT>(S, r, inv, q, st, ts)) return true;
+        } else {
+            if (opaque_cat<C_LATHE, COUNT>(S, r, inv, q, st, ts)) return true;
+            if (opaque_cat<C_PRISM, COUNT>(S, r, inv, q, st, ts)) return true;
+            if (S.csg_root >= 0 && csg()) return true;
+        }
     }
     bool any = false;
     float closest = r.tmax;

@@ -502,6 +502,22 @@ static SheenContext* context_create(const SheenScene* in, int flags = 0) {
         sheen_set_error("CSG leaves index at most 2^30 spheres or superellipsoids");  // hit records carry 30 bits
         return nullptr;
     }
+    {
+        // opacity of the CSG leaves' materials, as cat_opaque for the BVH categories
+        float maxtr = 0.0f;
+        for (int i = 0; d.csg && i < in->num_csg_nodes; i++) {
+            const CSGNode& n = in->csg_nodes[i];
+            if (n.leaf_kind < 0) continue;
+            int m = -1;
+            if (n.leaf_kind == 0 && n.leaf_index >= 0 && n.leaf_index < in->num_objects) m = in->objects[n.leaf_index].material_id;
+            if (n.leaf_kind == 1 && n.leaf_index >= 0 && n.leaf_index < in->num_superellipsoids)
+                m = in->superellipsoids[n.leaf_index].material_id;
+            maxtr = (m >= 0 && m < in->num_materials) ? std::max(maxtr, in->materials[m].transmission) : 1.0f;
+        }
+        const bool lathe_ok = d.count[C_LATHE] == 0 || d.cat_opaque[C_LATHE];
+        const bool prism_ok = d.count[C_PRISM] == 0 || d.cat_opaque[C_PRISM];
+        d.opaque_tail = (lathe_ok && prism_ok && maxtr <= 0.5f) ? 1 : 0;
+    }
     if (d.csg) {
         std::vector<float> cbox;
         host_csg_boxes(*in, cbox);
