@@ -549,6 +549,14 @@ SHD void csg_eval(const DScene& S, const DRay& r, v3 inv, float t0, float t1, in
             continue;
         }
         if (f.state == 0) {  // evaluate left
+            if (n.op == 1 && !(n.right >= 0 && top + 1 < frames && visit(n.right))) {
+                // an intersection whose right operand contributes nothing
+                // (its box missed, or no frame left for it) is empty, so the
+                // left subtree is not evaluated either
+                nret = 0;
+                top--;
+                continue;
+            }
             f.state = 1;
             if (n.left >= 0 && top + 1 < frames && visit(n.left)) {
                 fr[top + 1].node = n.left; fr[top + 1].state = 0; fr[top + 1].nA = 0;
