@@ -769,7 +769,10 @@ SHD void lathe_scan(const DScene& S, const DRay& r, v3 inv, float& ct, int& ci, 
 template <int CAT, bool ANY, bool COUNT>
 SHD void cat_traverse(const DScene& S, const DRay& r, v3 inv, float& ct, int& ci, bool& ch, TStack& st,
                       TraceStats& ts) {
-    if (CAT == C_LATHE && S.lathe_scan > 0) lathe_scan<ANY, COUNT>(S, r, inv, ct, ci, ch, ts);
+    // the scan reads the lathes through the scalar cache: every active lane
+    // must hold this descriptor (a batch wave whose lanes trace several
+    // scenes traverses the trees)
+    if (CAT == C_LATHE && S.lathe_scan > 0 && S.wave_one_scene) lathe_scan<ANY, COUNT>(S, r, inv, ct, ci, ch, ts);
     else traverse<CAT, ANY, COUNT>(S, r, inv, ct, ci, ch, st, ts);
 }
 

@@ -31,8 +31,8 @@ CAMERAS = [  # position, forward, up
 ]
 
 
-def _scene(cam, n_lathes=8):
-    rng = np.random.default_rng(5)
+def _scene(cam, n_lathes=8, seed=5):
+    rng = np.random.default_rng(seed)
     ps = PyScene()
     ps.set("materials", K.Material, [material(color=(0.8, 0.7, 0.6), spec=0.0), material(color=(0.3, 0.6, 0.9))],
            "num_materials")
@@ -87,3 +87,17 @@ def test_lathe_hits_match_oracle(cam, n_lathes):
     assert np.array_equal(t.view(np.uint32), ref["t"].view(np.uint32))
     rel = np.abs(img[..., :3] - ref["image"][..., :3]) / np.maximum(np.abs(ref["image"][..., :3]), 1e-3)
     assert float(np.mean(rel.max(-1) > 1e-4)) <= 0.001, rel.max()
+
+
+def test_lathe_scenes_batched_equal_their_own_renders():
+    """Three lathe scenes in one batch: the shadow and secondary queues mix
+    their rays in a wave, where the lathes are traversed through each scene's
+    tree; each batched image equals the scene's own render (scanned lathes)."""
+    from sheen_amd.batch import Batch
+    sc = [_scene(CAMERAS[0], 8, seed=5), _scene(CAMERAS[2], 12, seed=6), _scene(CAMERAS[1], 5, seed=7)]
+    sizes = [(72, 56, 2), (64, 48, 3), (40, 40, 2)]
+    with Batch([s.ptr for s in sc], sizes) as b:
+        b.render()
+        b.render_serial()
+        for k in range(b.n):
+            assert np.array_equal(b.image(k), b.image(k, serial=True)), k
