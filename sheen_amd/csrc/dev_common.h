@@ -114,6 +114,7 @@ struct DScene {
     const Lathe* lathes;
     const LathePoint* lathe_points;
     const LatheFrustum* lathe_frusta;  // per profile point: the frustum to the next point (lathe_frustum)
+    const float2* lathe_yr;            // per lathe: lowest and highest profile y of its frusta (+-inf: any)
     // lathes of the BVH in the order lathe_scan tests them, widest profile
     // first (lathe_scan of them; 0: the tree is traversed)
     const int* lathe_order;

@@ -753,6 +753,19 @@ SHD void lathe_scan(const DScene& S, const DRay& r, v3 inv, float& ct, int& ci, 
         const float mn[3] = {sload(bb), sload(bb + 1), sload(bb + 2)};
         const float mx[3] = {sload(bb + 3), sload(bb + 4), sload(bb + 5)};
         if (!ref_slab(r, inv, mn, mx)) continue;
+        // The lathe is skipped when the ray's segment [t_min, running best]
+        // lies wholly above or below the y range of its frusta (lathe_yr): then
+        // every frustum's axial test would skip it (tp = +-(y - y0) beyond [0, L]
+        // at both ends).  The margin, 2^-16 of the magnitudes involved, exceeds
+        // by far the rounding that separates the float y here from the float tp
+        // of those tests (a few units of 2^-24 of the same magnitudes).
+        {
+            const float2 yr = sload(S.lathe_yr + idx);
+            const float ya = r.o.y + r.tmin * r.d.y, yb = r.o.y + ct * r.d.y;
+            const float ym = fmaxf(fabsf(yr.x), fabsf(yr.y)) + fabsf(r.o.y);
+            const float ma = 0x1p-16f * (ym + fabsf(r.tmin * r.d.y)), mb = 0x1p-16f * (ym + fabsf(ct * r.d.y));
+            if ((ya > yr.y + ma && yb > yr.y + mb) || (ya < yr.x - ma && yb < yr.x - mb)) continue;
+        }
         const Lathe l = sload(S.lathes + idx);
         if (l.point_count < 2) continue;
         if (COUNT) ts.prims++;

@@ -282,15 +282,28 @@ static_assert(sizeof(HostSpineNode) == sizeof(CsgSpineNode), "spine node layout"
 // Triangle leaf copies in leaf-slot order: v0 + index, v1, v2 (48 bytes).
 // Frustum records of the lathe profiles (lathe_frustum), one thread per lathe;
 // point ranges were validated on the host, and are clipped here as well.
-__global__ void k_lathe_frusta(const Lathe* lathes, int n, const LathePoint* pts, int n_pts, LatheFrustum* out) {
+__global__ void k_lathe_frusta(const Lathe* lathes, int n, const LathePoint* pts, int n_pts, LatheFrustum* out,
+                               float2* yr) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
     const Lathe L = lathes[l];
+    // the y range of the frusta that can be hit; a frustum taking the
+    // division path (an overflowing or NaN square) opens it to everything
+    float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
     for (int i = 0; i + 1 < L.point_count; i++) {
         const int g = L.point_offset + i;
         if (g < 0 || g + 1 >= n_pts) break;
-        out[g] = lathe_frustum(pts[g].y, pts[g].radius, pts[g + 1].y, pts[g + 1].radius);
+        const LatheFrustum f = lathe_frustum(pts[g].y, pts[g].radius, pts[g + 1].y, pts[g + 1].radius);
+        out[g] = f;
+        if (f.mode == 0) {
+            lo = fminf(lo, fminf(pts[g].y, pts[g + 1].y));
+            hi = fmaxf(hi, fmaxf(pts[g].y, pts[g + 1].y));
+        } else if (f.mode == 2) {
+            lo = -__builtin_huge_valf();
+            hi = __builtin_huge_valf();
+        }
     }
+    yr[l] = make_float2(lo, hi);  // (+inf, -inf): no frustum can be hit
 }
 __global__ void k_tri_leaf(const uint32_t* prims, int n, const Triangle* tris, float4* out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -487,12 +500,14 @@ static SheenContext* context_create(const SheenScene* in, int flags = 0) {
         std::vector<int> sorted((size_t)n);
         for (int j = 0; j < n; j++) sorted[j] = (int)ids[order[j]];
         LatheFrustum* fr = c->alloc_owned<LatheFrustum>((size_t)in->num_lathe_points);
+        float2* yr = c->alloc_owned<float2>((size_t)in->num_lathes);
         d.lathe_order = c->upload(sorted.data(), sorted.size());
-        if (!fr || !d.lathe_order) { sheen_set_error("lathe record upload failed"); return nullptr; }
+        if (!fr || !yr || !d.lathe_order) { sheen_set_error("lathe record upload failed"); return nullptr; }
         k_lathe_frusta<<<(in->num_lathes + 255) / 256, 256, 0, c->stream>>>(d.lathes, in->num_lathes, d.lathe_points,
-                                                                            in->num_lathe_points, fr);
+                                                                            in->num_lathe_points, fr, yr);
         HIP_OR(hipGetLastError(), nullptr);
         d.lathe_frusta = fr;
+        d.lathe_yr = yr;
         d.lathe_scan = n;
     }
     d.prisms = c->upload(in->prisms, in->prisms ? (size_t)in->num_prisms : 0);
