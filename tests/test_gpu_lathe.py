@@ -31,7 +31,7 @@ CAMERAS = [  # position, forward, up
 ]
 
 
-def _scene(cam, n_lathes=8, seed=5):
+def _scene(cam, n_lathes=8, seed=5, plane_at_top=False):
     rng = np.random.default_rng(seed)
     ps = PyScene()
     ps.set("materials", K.Material, [material(color=(0.8, 0.7, 0.6), spec=0.0), material(color=(0.3, 0.6, 0.9))],
@@ -68,15 +68,22 @@ def _scene(cam, n_lathes=8, seed=5):
         lathes.append(la)
     ps.set("lathe_points", K.LathePoint, pts, "num_lathe_points")
     ps.set("lathes", K.Lathe, lathes, "num_lathes")
+    if plane_at_top:
+        # the floor at the exact top of the first lathe's profile: segments
+        # that end on it stop at that height (the lathe's y range bound)
+        p.d = -float(np.float32(pts[lathes[0].point_count - 1].y))
+        ps.set("planes", K.Plane, [p], "num_planes")
     return ps
 
 
-@pytest.mark.parametrize("cam,n_lathes", [(c, 8) for c in range(len(CAMERAS))] + [(0, LATHE_SCAN_MAX), (2, LATHE_SCAN_MAX + 8)])
-def test_lathe_hits_match_oracle(cam, n_lathes):
-    """Scanned (up to LATHE_SCAN_MAX lathes) and tree-traversed lathes."""
+@pytest.mark.parametrize("cam,n_lathes,top", [(c, 8, False) for c in range(len(CAMERAS))] +
+                         [(0, LATHE_SCAN_MAX, False), (2, LATHE_SCAN_MAX + 8, False), (0, 8, True), (2, 8, True)])
+def test_lathe_hits_match_oracle(cam, n_lathes, top):
+    """Scanned (up to LATHE_SCAN_MAX lathes) and tree-traversed lathes, and a
+    floor at the top height of a lathe's profile."""
     from oracle import Oracle
     from sheen_amd import sheen
-    ps = _scene(CAMERAS[cam], n_lathes)
+    ps = _scene(CAMERAS[cam], n_lathes, plane_at_top=top)
     W, H, AA = 160, 120, 2
     with sheen.Context(ps.ptr) as ctx:
         img, _ = ctx.render(W, H, AA)
